@@ -1,0 +1,279 @@
+/*
+ * self_amd.h -- C ABI of the MI355X-native DGSEM tendency engine.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, else a positive status (see SELFAMD_E*);
+ *     selfamd_last_error() returns the message of the last failure on this thread.
+ *   - Pointers named *_dev are device pointers (hipMalloc'd) passed BY VALUE; every other
+ *     pointer is host memory.  The library never frees caller memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).  Calls only
+ *     enqueue work; no call synchronises the device except where stated.
+ *   - Arrays use the reference's Fortran column-major shapes (zero-based):
+ *       interior(0:N,0:N,nVar,nEl), vector interior(2,0:N,0:N,nVar,nEl),
+ *       tensor interior(2,2,0:N,0:N,nVar,nEl), boundary(0:N,nVar,4,nEl),
+ *       vector boundary(2,0:N,nVar,4,nEl), tensor boundary(2,2,0:N,nVar,4,nEl),
+ *       sideInfo(5,4,nEl) int32 with 1-based element ids.
+ *   - Reference interfaces are cited file:line relative to the SELF checkout.  The
+ *     reference wrappers take Fortran c_ptr arguments by reference (real**) and return
+ *     nothing; these entry points take device pointers by value, an explicit stream and
+ *     return a status.
+ */
+#ifndef SELF_AMD_H
+#define SELF_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SELFAMD_VERSION 1
+
+/* status codes */
+#define SELFAMD_OK 0
+#define SELFAMD_EBADARG 1
+#define SELFAMD_EUNSUPPORTED 2
+#define SELFAMD_EHIP 3
+#define SELFAMD_ENOTAFFINE 4
+
+/* quadrature codes (src/SELF_Constants.f90:78-84) */
+#define SELFAMD_GAUSS 1
+#define SELFAMD_GAUSS_LOBATTO 2
+#define SELFAMD_UNIFORM 5
+
+/* boundary condition ids (src/SELF_Model.f90:70-72) */
+#define SELFAMD_BC_PRESCRIBED 100
+#define SELFAMD_BC_RADIATION 101
+#define SELFAMD_BC_NONORMALFLOW 102
+
+/* physics models (src/models/SELF_LinearShallowWater.f90, SELF_ShallowWater.f90,
+ * SELF_CompressibleIdealGas2D.f90) */
+#define SELFAMD_MODEL_LSW 1
+#define SELFAMD_MODEL_SW 2
+#define SELFAMD_MODEL_EULER 3
+
+const char *selfamd_last_error(void);
+int selfamd_version(void);
+
+/* ------------------------------------------------------------------------------------ */
+/* Host setup                                                                            */
+/* ------------------------------------------------------------------------------------ */
+
+/* Replaces Init_Lagrange (src/SELF_Lagrange.f90:521-615) with LegendreGauss
+ * (src/SELF_Quadrature.f90:201-262).  Host arrays: controlPoints/bWeights/qWeights(0:N),
+ * targetPoints(0:M), iMatrix(0:N,0:M), dMatrix/dgMatrix(0:N,0:N), bMatrix(0:N,0:1). */
+int selfamd_lagrange_init(int N, int controlNodeType, int M, int targetNodeType,
+                          double *controlPoints, double *targetPoints, double *bWeights,
+                          double *qWeights, double *iMatrix, double *dMatrix, double *dgMatrix,
+                          double *bMatrix);
+
+/* Structured nx*ny quad mesh (nGeo = 1, raster element order).  No reference counterpart:
+ * the reference only reads HOPr files (src/SELF_Mesh.f90:852-1016); this emits the same
+ * sideInfo convention (src/SELF_Mesh.f90:84-91).  Periodic directions become interior
+ * sides.  Host outputs: sideInfo(5,4,nx*ny), nodeCoords(2,0:1,0:1,nx*ny),
+ * globalNodeIDs(0:1,0:1,nx*ny). */
+int selfamd_mesh2d_structured(int nx, int ny, double x0, double x1, double y0, double y1,
+                              int periodicX, int periodicY, int boundaryBcid, int32_t *sideInfo,
+                              double *nodeCoords, int32_t *globalNodeIDs);
+/* Replaces RecalculateFlip_Mesh2D (src/SELF_Mesh.f90:1018-1174), single rank. */
+int selfamd_mesh2d_recalculate_flip(int nElem, int nGeo, int32_t *sideInfo, const int32_t *globalNodeIDs);
+/* Replaces ResetBoundaryConditionType_Mesh2D (src/SELF_Mesh.f90:695-719). */
+int selfamd_mesh2d_reset_boundary_condition_type(int nElem, int32_t *sideInfo, int bcid);
+/* Replaces the 3-D -> 2-D part of Read_HOPr_Mesh2D (src/SELF_Mesh.f90:962-1007) on arrays
+ * already read from the file: sideInfo3D(5,6*nElem), nodeCoords3D(3,(nGeo+1)^3*nElem),
+ * globalNodeIDs3D((nGeo+1)^3*nElem). */
+int selfamd_mesh2d_from_hopr(int nElem, int nGeo, const int32_t *sideInfo3D, const double *nodeCoords3D,
+                             const int32_t *globalNodeIDs3D, int32_t *sideInfo, double *nodeCoords,
+                             int32_t *globalNodeIDs);
+
+/* ------------------------------------------------------------------------------------ */
+/* Lagrange / data operators (device)                                                    */
+/* ------------------------------------------------------------------------------------ */
+
+/* ScalarBoundaryInterp_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:393-399) */
+int selfamd_scalar_boundary_interp_2d(const double *bMatrix_dev, const double *f_dev, double *fBound_dev,
+                                      int N, int nVar, int nEl, void *stream);
+/* VectorBoundaryInterp_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:422-428) */
+int selfamd_vector_boundary_interp_2d(const double *bMatrix_dev, const double *f_dev, double *fBound_dev,
+                                      int N, int nVar, int nEl, void *stream);
+/* TensorBoundaryInterp_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp, TensorBoundaryInterp_2D) */
+int selfamd_tensor_boundary_interp_2d(const double *bMatrix_dev, const double *f_dev, double *fBound_dev,
+                                      int N, int nVar, int nEl, void *stream);
+/* ScalarGridInterp_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:85); iMatrix(0:N,0:M) */
+int selfamd_scalar_grid_interp_2d(const double *iMatrix_dev, const double *f_dev, double *fNew_dev,
+                                  int N, int M, int nVar, int nEl, void *stream);
+/* VectorGridInterp_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:114) */
+int selfamd_vector_grid_interp_2d(const double *iMatrix_dev, const double *f_dev, double *fNew_dev,
+                                  int N, int M, int nVar, int nEl, void *stream);
+/* VectorGradient_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:707) */
+int selfamd_vector_gradient_2d(const double *dMatrix_dev, const double *f_dev, double *gradF_dev,
+                               int N, int nVar, int nEl, void *stream);
+/* VectorDivergence_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:777), strong form */
+int selfamd_vector_divergence_2d(const double *dMatrix_dev, const double *f_dev, double *dF_dev,
+                                 int N, int nVar, int nEl, void *stream);
+/* VectorDGDivergence_2D_gpu_wrapper (src/hip/SELF_Lagrange.cpp:825-831), weak form */
+int selfamd_vector_dg_divergence_2d(const double *dgMatrix_dev, const double *bMatrix_dev,
+                                    const double *qWeights_dev, const double *f_dev, const double *bF_dev,
+                                    double *dF_dev, int N, int nVar, int nEl, void *stream);
+/* Determinant_Tensor2D_gpu_wrapper (src/hip/SELF_Data.cpp, Determinant_Tensor2D) */
+int selfamd_determinant_tensor_2d(const double *t_dev, double *det_dev, int N, int nVar, int nEl, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Mapped operators and model register updates (device)                                  */
+/* ------------------------------------------------------------------------------------ */
+
+/* SideExchange_MappedScalar2D_gpu_wrapper (src/hip/SELF_MappedData.cpp:941-947); sideInfo
+ * is a device copy.  Only sides with bcid == 0 are written (src/SELF_MappedData.f90:878). */
+int selfamd_side_exchange_scalar_2d(const double *boundary_dev, double *extBoundary_dev,
+                                    const int32_t *sideInfo_dev, int N, int nVar, int nEl, void *stream);
+/* SideExchange_MappedVector2D_gpu_wrapper (src/hip/SELF_MappedData.cpp:950) */
+int selfamd_side_exchange_vector_2d(const double *boundary_dev, double *extBoundary_dev,
+                                    const int32_t *sideInfo_dev, int N, int nVar, int nEl, void *stream);
+/* GradientSF_MappedScalar2D_gpu_wrapper (src/hip/SELF_MappedData.cpp:171); dsdx and J are
+ * the geometry arrays (nVar = 1). */
+int selfamd_gradient_sf_2d(const double *dMatrix_dev, const double *dsdx_dev, const double *J_dev,
+                           const double *f_dev, double *gradF_dev, int N, int nVar, int nEl, void *stream);
+/* JacobianWeight_MappedScalar2D_gpu_wrapper (src/hip/SELF_MappedData.cpp:210), in place */
+int selfamd_jacobian_weight_2d(const double *J_dev, double *f_dev, int N, int nVar, int nEl, void *stream);
+/* ContravariantProjection_MappedVector2D_gpu_wrapper (src/hip/SELF_MappedData.cpp:401), in place */
+int selfamd_contravariant_projection_2d(const double *dsdx_dev, double *f_dev, int N, int nVar, int nEl,
+                                        void *stream);
+/* UpdateGRK_Model2D_gpu_wrapper (src/hip/SELF_Model2D.cpp:153-157):
+ * G = a*G + dSdt ; S = S + (g*dt)*G */
+int selfamd_update_grk(double *G_dev, double *S_dev, const double *dSdt_dev, double a, double g, double dt,
+                       int N, int nVar, int nEl, void *stream);
+/* CalculateDSDt_Model2D_gpu_wrapper (src/hip/SELF_Model2D.cpp:192-196): dSdt = source - fluxDiv */
+int selfamd_calculate_dsdt(const double *fluxDiv_dev, const double *source_dev, double *dSdt_dev, int N,
+                           int nVar, int nEl, void *stream);
+
+/* GenerateFromMesh_SEMQuad + CalculateMetricTerms_SEMQuad (src/SELF_Geometry.f90:393-552),
+ * computed on the device (the reference runs these on the host; its GPU branch is
+ * commented out at :419-429,532-543).  nodeCoords(2,0:nGeo,0:nGeo,nEl) on uniform control
+ * points; outputs are SEMQuad's x, dxds, dsdx, nHat, nScale, J in the shapes above
+ * (interior and boundary).  Synchronises `stream` before returning. */
+int selfamd_semquad_generate(int N, int controlNodeType, int nGeo, int nEl, const double *nodeCoords_dev,
+                             double *x_dev, double *xb_dev, double *dxds_dev, double *dxdsb_dev,
+                             double *dsdx_dev, double *dsdxb_dev, double *nHatb_dev, double *nScaleb_dev,
+                             double *J_dev, double *Jb_dev, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Physics operators (device), one per reference plug-in method                          */
+/* ------------------------------------------------------------------------------------ */
+
+/* SW PreTendency (src/models/SELF_ShallowWater.f90:190-235): velocity interior from the
+ * current interior, velocity boundary from the solution boundary as it stands (the
+ * previous stage's traces).  Follow with selfamd_side_exchange_vector_2d. */
+int selfamd_sw2d_velocity(const double *sol_dev, const double *solB_dev, double *vel_dev, double *velB_dev,
+                          int N, int nEl, void *stream);
+/* SetBoundaryCondition_ShallowWater (src/models/SELF_ShallowWater.f90:284-343) */
+int selfamd_sw2d_set_boundary_condition(const double *solB_dev, double *solE_dev, const double *topoB_dev,
+                                        const double *nHatb_dev, const int32_t *sideInfo_dev, int N, int nEl,
+                                        void *stream);
+/* Source_ShallowWater (src/models/SELF_ShallowWater.f90:345-384) */
+int selfamd_sw2d_source(double *source_dev, const double *sol_dev, const double *gradTopo_dev, double g,
+                        double fCori, int N, int nEl, void *stream);
+/* Flux_ShallowWater (src/models/SELF_ShallowWater.f90:386-449) */
+int selfamd_sw2d_flux(double *flux_dev, const double *sol_dev, const double *vel_dev, double g, int N, int nEl,
+                      void *stream);
+/* RiemannSolver_ShallowWater (src/models/SELF_ShallowWater.f90:451-529), Rusanov */
+int selfamd_sw2d_riemann(double *fluxBN_dev, const double *solB_dev, const double *solE_dev,
+                         const double *velB_dev, const double *velE_dev, const double *nHatb_dev,
+                         const double *nScaleb_dev, double g, int N, int nEl, void *stream);
+
+/* SetBoundaryCondition_LinearShallowWater_gpu_wrapper (src/models/hip/SELF_LinearShallowWater.cpp:6),
+ * restated from the CPU branch (src/models/SELF_LinearShallowWater.f90:298-355) */
+int selfamd_lsw2d_set_boundary_condition(const double *solB_dev, double *solE_dev, const double *nHatb_dev,
+                                         const int32_t *sideInfo_dev, int N, int nEl, void *stream);
+/* Source_LinearShallowWater_gpu_wrapper (src/models/hip/SELF_LinearShallowWater.cpp:54) */
+int selfamd_lsw2d_source(double *source_dev, const double *sol_dev, const double *fCori_dev, int N, int nEl,
+                         void *stream);
+/* Flux_LinearShallowWater_gpu_wrapper (src/models/hip/SELF_LinearShallowWater.cpp:80) */
+int selfamd_lsw2d_flux(double *flux_dev, const double *sol_dev, const double *H_dev, double g, int N, int nEl,
+                       void *stream);
+/* RiemannSolver_LinearShallowWater_gpu_wrapper (src/models/hip/SELF_LinearShallowWater.cpp:108) */
+int selfamd_lsw2d_riemann(double *fluxBN_dev, const double *solB_dev, const double *solE_dev,
+                          const double *Hb_dev, const double *nHatb_dev, const double *nScaleb_dev, double g,
+                          int N, int nEl, void *stream);
+
+/* CalculateDiagnostics_CompressibleIdealGas2D_gpu_wrapper (src/models/hip/SELF_CompressibleIdealGas2D.cpp:324) */
+int selfamd_euler2d_diagnostics(const double *sol_dev, double *diag_dev, double gamma, double R, int N, int nEl,
+                                void *stream);
+/* ConservativeToPrimitive_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:376) */
+int selfamd_euler2d_primitive(const double *sol_dev, double *prim_dev, double gamma, int N, int nEl, void *stream);
+/* ConservativeToEntropy_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:413) */
+int selfamd_euler2d_entropy_vars(const double *sol_dev, double *ent_dev, double gamma, int N, int nEl,
+                                 void *stream);
+/* SetBoundaryCondition_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:208),
+ * restated from the CPU branch (src/models/SELF_CompressibleIdealGas2D.f90:1217-1323) */
+int selfamd_euler2d_set_boundary_condition(const double *solB_dev, double *solE_dev, const double *presSolB_dev,
+                                           const double *primB_dev, double *primE_dev,
+                                           const double *presPrimB_dev, const double *diagB_dev,
+                                           double *diagE_dev, const double *presDiagB_dev,
+                                           const double *nHatb_dev, const int32_t *sideInfo_dev, int N, int nEl,
+                                           void *stream);
+/* Source_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:5), restated from
+ * the CPU branch, drag term included (src/models/SELF_CompressibleIdealGas2D.f90:1325-1366).
+ * env(0:N,0:N,2,nEl) = (potential, drag); envGrad(2,0:N,0:N,2,nEl). */
+int selfamd_euler2d_source(double *source_dev, const double *sol_dev, const double *env_dev,
+                           const double *envGrad_dev, int N, int nEl, void *stream);
+/* Flux_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:34) */
+int selfamd_euler2d_flux(double *flux_dev, const double *sol_dev, const double *prim_dev, const double *diag_dev,
+                         int N, int nEl, void *stream);
+/* NaiveLLF_CompressibleIdealGas2D_gpu_wrapper (...CompressibleIdealGas2D.cpp:101) */
+int selfamd_euler2d_riemann(double *fluxBN_dev, const double *solB_dev, const double *solE_dev,
+                            const double *primB_dev, const double *primE_dev, const double *diagB_dev,
+                            const double *diagE_dev, const double *nHatb_dev, const double *nScaleb_dev, int N,
+                            int nEl, void *stream);
+
+/* ------------------------------------------------------------------------------------ */
+/* Model2D engine: the fused tendency + low-storage RK3 path                             */
+/* ------------------------------------------------------------------------------------ */
+/* One handle replaces a Model2D extension (src/SELF_Model2D.f90:18-71) on one GPU:
+ * CalculateTendency (:816-859) and ForwardStep/LowStorageRK3 (src/SELF_Model.f90:474-523,
+ * 704-732) run as one fused kernel per RK stage.  The state stays resident on the device. */
+typedef struct selfamd_model2d selfamd_model2d;
+
+/* scalar parameters (selfamd_model2d_set_param) */
+#define SELFAMD_PARAM_G 1      /* gravity g (LSW, SW) */
+#define SELFAMD_PARAM_FCORI 2  /* SW constant Coriolis parameter */
+#define SELFAMD_PARAM_GAMMA 3  /* Euler expansion factor Cp/Cv */
+#define SELFAMD_PARAM_RGAS 4   /* Euler gas constant */
+/* static per-node fields (selfamd_model2d_set_field), interior(0:N,0:N,1,nEl) device arrays */
+#define SELFAMD_FIELD_TOPOGRAPHY 1  /* SW this%H (SetTopography) */
+#define SELFAMD_FIELD_BATHYMETRY 2  /* LSW this%H (SetBathymetry) */
+#define SELFAMD_FIELD_CORIOLIS 3    /* LSW this%fCori (SetCoriolis) */
+#define SELFAMD_FIELD_POTENTIAL 4   /* Euler gravity potential (SetGravity) */
+#define SELFAMD_FIELD_DRAG 5        /* Euler drag coefficient (SetDrag) */
+
+/* Init_Model2D + the model's Init (e.g. src/models/SELF_ShallowWater.f90:96-147).  sideInfo is
+ * host memory; geometry arrays are SEMQuad device arrays (dsdx interior, J interior, nHat and
+ * nScale boundary) from selfamd_semquad_generate.  Fails with SELFAMD_ENOTAFFINE when the
+ * metric terms are not constant per element (curved meshes run through the operator API). */
+int selfamd_model2d_create(selfamd_model2d **model, int kind, int N, int nEl, const int32_t *sideInfo,
+                           const double *dsdx_dev, const double *J_dev, const double *nHatb_dev,
+                           const double *nScaleb_dev, void *stream);
+int selfamd_model2d_destroy(selfamd_model2d *model);
+int selfamd_model2d_set_param(selfamd_model2d *model, int which, double value);
+int selfamd_model2d_set_field(selfamd_model2d *model, int which, const double *field_dev);
+/* SetSolution (src/SELF_Model2D.f90:313-332): copies interior(0:N,0:N,nVar,nEl) and
+ * interpolates it to the element boundaries. */
+int selfamd_model2d_set_solution(selfamd_model2d *model, const double *interior_dev);
+int selfamd_model2d_get_solution(selfamd_model2d *model, double *interior_dev);
+/* CalculateTendency (src/SELF_Model2D.f90:816-859): dSdt(0:N,0:N,nVar,nEl) */
+int selfamd_model2d_tendency(selfamd_model2d *model, double *dSdt_dev);
+/* nSteps full low-storage RK3 steps of size dt (ForwardStep with tn = t + nSteps*dt). */
+int selfamd_model2d_forward_step(selfamd_model2d *model, int nSteps, double dt);
+/* Blocks until the model's stream has drained. */
+int selfamd_model2d_synchronize(selfamd_model2d *model);
+/* forward_step bracketed by two hipEvents on the model's stream; returns the elapsed
+ * milliseconds between them (synchronises). */
+int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, double *elapsedMs);
+/* Model time t */
+int selfamd_model2d_get_time(selfamd_model2d *model, double *t);
+int selfamd_model2d_set_time(selfamd_model2d *model, double t);
+/* Number of conserved variables (3 for LSW/SW, 4 for Euler). */
+int selfamd_model2d_nvar(int kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,308 @@
+"""self_amd -- MI355X-native DGSEM tendency + low-storage RK3 engine.
+
+Host-side mirror of the SELF model API for the 2D DGSEM path (Lagrange, Mesh2D, SEMQuad,
+Model2D and the LinearShallowWater / ShallowWater / CompressibleIdealGas2D plug-ins).  All
+numerics run in libself_amd.so (hand-written HIP kernels for gfx950) through its C ABI;
+PyTorch only provides device memory.  There is no CPU fallback: without the native
+library or a GPU the compute calls raise.
+
+Array convention: a Fortran array a(0:N,0:N,nVar,nEl) is held in C order with the shape
+reversed, a[e, v, j, i]; boundary(0:N,nVar,4,nEl) is b[e, s, v, i]; sideInfo(5,4,nEl) is
+si[e, s, k].
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SelfAmdError, call, load
+
+GAUSS, GAUSS_LOBATTO, UNIFORM = 1, 2, 5
+BC_PRESCRIBED, BC_RADIATION, BC_NONORMALFLOW = 100, 101, 102
+MODEL_LSW, MODEL_SW, MODEL_EULER = 1, 2, 3
+PARAM_G, PARAM_FCORI, PARAM_GAMMA, PARAM_RGAS = 1, 2, 3, 4
+FIELD_TOPOGRAPHY, FIELD_BATHYMETRY, FIELD_CORIOLIS, FIELD_POTENTIAL, FIELD_DRAG = 1, 2, 3, 4, 5
+
+__all__ = ["Lagrange", "Mesh2D", "SEMQuad", "Model2D", "ShallowWater", "LinearShallowWater",
+           "CompressibleIdealGas2D", "SelfAmdError", "load", "GAUSS", "GAUSS_LOBATTO", "UNIFORM",
+           "BC_PRESCRIBED", "BC_RADIATION", "BC_NONORMALFLOW"]
+
+
+def _ptr(a):
+    """Address of a numpy array (host) or torch tensor (device) for the C ABI."""
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return ctypes.c_void_p(a.ctypes.data)
+    assert a.is_contiguous()
+    return ctypes.c_void_p(a.data_ptr())
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise SelfAmdError("self_amd compute calls need a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _dev(x, dtype=None):
+    """Contiguous float64 (or given dtype) device tensor from numpy or torch input."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    if isinstance(x, np.ndarray):
+        return torch.from_numpy(np.ascontiguousarray(x)).to(device="cuda", dtype=dtype)
+    return x.to(device="cuda", dtype=dtype).contiguous()
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Lagrange:
+    """Lagrange interpolant on N+1 control points (Init_Lagrange, src/SELF_Lagrange.f90:521-615).
+
+    Host numpy arrays: controlPoints, targetPoints, bWeights, qWeights, iMatrix[M+1, N+1]
+    (= Fortran iMatrix(0:N,0:M)), dMatrix, dgMatrix [N+1, N+1] (Fortran (0:N,0:N)) and
+    bMatrix[2, N+1] (Fortran (0:N,0:1)).
+    """
+
+    def __init__(self, N, controlNodeType=GAUSS, M=None, targetNodeType=UNIFORM):
+        M = N if M is None else M
+        self.N, self.M = N, M
+        self.controlNodeType, self.targetNodeType = controlNodeType, targetNodeType
+        P, Q = N + 1, M + 1
+        self.controlPoints = np.zeros(P)
+        self.targetPoints = np.zeros(Q)
+        self.bWeights = np.zeros(P)
+        self.qWeights = np.zeros(P)
+        self.iMatrix = np.zeros((Q, P))
+        self.dMatrix = np.zeros((P, P))
+        self.dgMatrix = np.zeros((P, P))
+        self.bMatrix = np.zeros((2, P))
+        call("selfamd_lagrange_init", N, controlNodeType, M, targetNodeType,
+             *[_ptr(a) for a in (self.controlPoints, self.targetPoints, self.bWeights, self.qWeights,
+                                  self.iMatrix, self.dMatrix, self.dgMatrix, self.bMatrix)])
+        self._dev = None
+
+    def device(self):
+        """Device copies (iMatrix, dMatrix, dgMatrix, bMatrix, qWeights) as torch tensors."""
+        if self._dev is None:
+            self._dev = {k: _dev(getattr(self, k)) for k in ("iMatrix", "dMatrix", "dgMatrix", "bMatrix", "qWeights")}
+        return self._dev
+
+
+class Mesh2D:
+    """2-D quad mesh connectivity in the reference convention (src/SELF_Mesh.f90:27-91)."""
+
+    def __init__(self, sideInfo, nodeCoords, globalNodeIDs, nGeo, quadrature=UNIFORM):
+        self.sideInfo = np.ascontiguousarray(sideInfo, dtype=np.int32)        # [nEl, 4, 5]
+        self.nodeCoords = np.ascontiguousarray(nodeCoords, dtype=np.float64)  # [nEl, G, G, 2]
+        self.globalNodeIDs = np.ascontiguousarray(globalNodeIDs, dtype=np.int32)  # [nEl, G, G]
+        self.nGeo = nGeo
+        self.nElem = self.sideInfo.shape[0]
+        self.quadrature = quadrature
+
+    @classmethod
+    def structured(cls, nx, ny, x0=0.0, x1=1.0, y0=0.0, y1=1.0, periodic=(False, False), bcid=BC_RADIATION):
+        n = nx * ny
+        si = np.zeros((n, 4, 5), np.int32)
+        nc = np.zeros((n, 2, 2, 2))
+        gn = np.zeros((n, 2, 2), np.int32)
+        call("selfamd_mesh2d_structured", nx, ny, x0, x1, y0, y1, int(periodic[0]), int(periodic[1]), bcid,
+             _ptr(si), _ptr(nc), _ptr(gn))
+        m = cls(si, nc, gn, 1)
+        m.nx, m.ny = nx, ny
+        return m
+
+    @classmethod
+    def from_hopr(cls, nGeo, sideInfo3D, nodeCoords3D, globalNodeIDs3D):
+        """Read_HOPr_Mesh2D's 3-D -> 2-D conversion on arrays read from a HOPr file."""
+        si3 = np.ascontiguousarray(sideInfo3D, np.int32)
+        nEl = si3.shape[0] // 6
+        G = nGeo + 1
+        si = np.zeros((nEl, 4, 5), np.int32)
+        nc = np.zeros((nEl, G, G, 2))
+        gn = np.zeros((nEl, G, G), np.int32)
+        call("selfamd_mesh2d_from_hopr", nEl, nGeo, _ptr(si3), _ptr(np.ascontiguousarray(nodeCoords3D, np.float64)),
+             _ptr(np.ascontiguousarray(globalNodeIDs3D, np.int32)), _ptr(si), _ptr(nc), _ptr(gn))
+        return cls(si, nc, gn, nGeo)
+
+    def RecalculateFlip(self):
+        call("selfamd_mesh2d_recalculate_flip", self.nElem, self.nGeo, _ptr(self.sideInfo), _ptr(self.globalNodeIDs))
+
+    def ResetBoundaryConditionType(self, bcid):
+        call("selfamd_mesh2d_reset_boundary_condition_type", self.nElem, _ptr(self.sideInfo), bcid)
+
+
+class SEMQuad:
+    """Geometry (metric terms) generated on the device (GenerateFromMesh_SEMQuad,
+    src/SELF_Geometry.f90:393-552).  Attributes are device tensors in the C-order views:
+    x[e,j,i,2], xb[e,s,i,2], dxds/dsdx[e,j,i,2(c),2(r)], dsdxb[e,s,i,2,2], nHatb[e,s,i,2],
+    nScaleb[e,s,i], J[e,j,i], Jb[e,s,i]."""
+
+    def __init__(self, interp, mesh):
+        torch = _torch()
+        N, nEl, P = interp.N, mesh.nElem, interp.N + 1
+        self.interp, self.nElem = interp, nEl
+        f64 = dict(dtype=torch.float64, device="cuda")
+        self.x = torch.empty((nEl, P, P, 2), **f64)
+        self.xb = torch.empty((nEl, 4, P, 2), **f64)
+        self.dxds = torch.empty((nEl, P, P, 2, 2), **f64)
+        self.dxdsb = torch.empty((nEl, 4, P, 2, 2), **f64)
+        self.dsdx = torch.empty((nEl, P, P, 2, 2), **f64)
+        self.dsdxb = torch.empty((nEl, 4, P, 2, 2), **f64)
+        self.nHatb = torch.empty((nEl, 4, P, 2), **f64)
+        self.nScaleb = torch.empty((nEl, 4, P), **f64)
+        self.J = torch.empty((nEl, P, P), **f64)
+        self.Jb = torch.empty((nEl, 4, P), **f64)
+        nc = _dev(mesh.nodeCoords)
+        torch.cuda.synchronize()
+        call("selfamd_semquad_generate", N, interp.controlNodeType, mesh.nGeo, nEl, _ptr(nc),
+             *[_ptr(t) for t in (self.x, self.xb, self.dxds, self.dxdsb, self.dsdx, self.dsdxb, self.nHatb,
+                                 self.nScaleb, self.J, self.Jb)], _stream())
+
+    def CovariantArcMin(self):
+        """CovariantArcMin_SEMQuad (src/SELF_Geometry.f90:554-582)."""
+        d = self.dxds.cpu().numpy()  # [e, j, i, c, r] = dxds(r, c)
+        dx = np.sqrt(d[..., 0, 0] ** 2 + d[..., 1, 0] ** 2)  # sqrt(dxds(1,1)^2 + dxds(1,2)^2)
+        dy = np.sqrt(d[..., 0, 1] ** 2 + d[..., 1, 1] ** 2)  # sqrt(dxds(2,1)^2 + dxds(2,2)^2)
+        N = self.interp.N
+        return float(np.min(2.0 * np.minimum(dx, dy) / (N * N)))
+
+
+class Model2D:
+    """Model2D + physics plug-in on one GPU (src/SELF_Model2D.f90:18-71).  The state stays
+    resident in the engine; CalculateTendency and ForwardStep run the fused HIP kernels."""
+
+    kind = None
+
+    def __init__(self, mesh, geometry):
+        torch = _torch()
+        self.mesh, self.geometry = mesh, geometry
+        self.N = geometry.interp.N
+        self.nElem = mesh.nElem
+        self.nVar = load().selfamd_model2d_nvar(self.kind)
+        self._h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        call("selfamd_model2d_create", ctypes.byref(self._h), self.kind, self.N, self.nElem, _ptr(mesh.sideInfo),
+             _ptr(geometry.dsdx), _ptr(geometry.J), _ptr(geometry.nHatb), _ptr(geometry.nScaleb), None)
+        self.dt = 0.0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().selfamd_model2d_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def shape(self):
+        P = self.N + 1
+        return (self.nElem, self.nVar, P, P)
+
+    def _param(self, which, value):
+        call("selfamd_model2d_set_param", self._h, which, float(value))
+
+    def _field(self, which, values):
+        torch = _torch()
+        f = _dev(values).reshape(self.nElem, self.N + 1, self.N + 1).contiguous()
+        torch.cuda.synchronize()
+        call("selfamd_model2d_set_field", self._h, which, _ptr(f))
+
+    def SetSolution(self, interior):
+        torch = _torch()
+        s = _dev(interior).reshape(self.shape).contiguous()
+        torch.cuda.synchronize()
+        call("selfamd_model2d_set_solution", self._h, _ptr(s))
+
+    @property
+    def solution(self):
+        torch = _torch()
+        out = torch.empty(self.shape, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        call("selfamd_model2d_get_solution", self._h, _ptr(out))
+        return out
+
+    def CalculateTendency(self):
+        torch = _torch()
+        out = torch.empty(self.shape, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        call("selfamd_model2d_tendency", self._h, _ptr(out))
+        return out
+
+    def ForwardStep(self, nSteps=1, dt=None):
+        """nSteps low-storage RK3 steps (LowStorageRK3_timeIntegrator, src/SELF_Model.f90:704-732)."""
+        if dt is not None:
+            self.dt = dt
+        call("selfamd_model2d_forward_step", self._h, int(nSteps), float(self.dt))
+        call("selfamd_model2d_synchronize", self._h)
+
+    def time_steps(self, nSteps, dt):
+        """Run nSteps and return the hipEvent-measured milliseconds on the model's stream."""
+        ms = ctypes.c_double()
+        call("selfamd_model2d_time_steps", self._h, int(nSteps), float(dt), ctypes.byref(ms))
+        return ms.value
+
+    @property
+    def t(self):
+        v = ctypes.c_double()
+        call("selfamd_model2d_get_time", self._h, ctypes.byref(v))
+        return v.value
+
+    @t.setter
+    def t(self, value):
+        call("selfamd_model2d_set_time", self._h, float(value))
+
+
+class ShallowWater(Model2D):
+    """Nonlinear shallow water (src/models/SELF_ShallowWater.f90): state (Hu, Hv, H)."""
+    kind = MODEL_SW
+
+    def __init__(self, mesh, geometry, g=1.0, fCori=0.0):
+        super().__init__(mesh, geometry)
+        self._param(PARAM_G, g)
+        self._param(PARAM_FCORI, fCori)
+
+    def SetTopography(self, topo):
+        self._field(FIELD_TOPOGRAPHY, topo)
+
+
+class LinearShallowWater(Model2D):
+    """Linear shallow water (src/models/SELF_LinearShallowWater.f90): state (u, v, eta)."""
+    kind = MODEL_LSW
+
+    def __init__(self, mesh, geometry, g=1.0):
+        super().__init__(mesh, geometry)
+        self._param(PARAM_G, g)
+
+    def SetBathymetry(self, H):
+        if np.isscalar(H):
+            H = np.full((self.nElem, self.N + 1, self.N + 1), float(H))
+        self._field(FIELD_BATHYMETRY, H)
+
+    def SetCoriolis(self, f):
+        if np.isscalar(f):
+            f = np.full((self.nElem, self.N + 1, self.N + 1), float(f))
+        self._field(FIELD_CORIOLIS, f)
+
+
+class CompressibleIdealGas2D(Model2D):
+    """Compressible ideal gas (src/models/SELF_CompressibleIdealGas2D.f90): state
+    (rho u, rho v, rho, rho E).  Boundary sides with bcid 100/101 use the prescribed state
+    taken from the last SetSolution (SetPrescribedSolutionFromSolution)."""
+    kind = MODEL_EULER
+
+    def __init__(self, mesh, geometry, gamma=1.4, R=1.0):
+        super().__init__(mesh, geometry)
+        self._param(PARAM_GAMMA, gamma)
+        self._param(PARAM_RGAS, R)
+
+    def SetGravity(self, phi):
+        self._field(FIELD_POTENTIAL, phi)
+
+    def SetDrag(self, cd):
+        if np.isscalar(cd):
+            cd = np.full((self.nElem, self.N + 1, self.N + 1), float(cd))
+        self._field(FIELD_DRAG, cd)

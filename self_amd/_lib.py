@@ -1,0 +1,109 @@
+"""ctypes binding of libself_amd.so (the C ABI declared in include/self_amd.h).
+
+The library is built in-tree (``self_amd/libself_amd.so``) by ``__graft_entry__.build()``
+or ``make -C self_amd/csrc``.  Loading fails loudly when it is missing: there is no
+fallback implementation.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libself_amd.so")
+
+c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+# (name, argtypes) for every exported entry point; the header is the source of truth and
+# tests/test_abi.py checks that each declared symbol is listed and exported.
+SIGNATURES = {
+    "selfamd_last_error": [],
+    "selfamd_version": [],
+    "selfamd_lagrange_init": [c_int, c_int, c_int, c_int] + [c_void_p] * 8,
+    "selfamd_mesh2d_structured": [c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
+                                  c_void_p, c_void_p, c_void_p],
+    "selfamd_mesh2d_recalculate_flip": [c_int, c_int, c_void_p, c_void_p],
+    "selfamd_mesh2d_reset_boundary_condition_type": [c_int, c_void_p, c_int],
+    "selfamd_mesh2d_from_hopr": [c_int, c_int] + [c_void_p] * 6,
+    "selfamd_scalar_boundary_interp_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_vector_boundary_interp_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_tensor_boundary_interp_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_scalar_grid_interp_2d": [c_void_p] * 3 + [c_int] * 4 + [c_void_p],
+    "selfamd_vector_grid_interp_2d": [c_void_p] * 3 + [c_int] * 4 + [c_void_p],
+    "selfamd_vector_gradient_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_vector_divergence_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_vector_dg_divergence_2d": [c_void_p] * 6 + [c_int] * 3 + [c_void_p],
+    "selfamd_determinant_tensor_2d": [c_void_p] * 2 + [c_int] * 3 + [c_void_p],
+    "selfamd_side_exchange_scalar_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_side_exchange_vector_2d": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_gradient_sf_2d": [c_void_p] * 5 + [c_int] * 3 + [c_void_p],
+    "selfamd_jacobian_weight_2d": [c_void_p] * 2 + [c_int] * 3 + [c_void_p],
+    "selfamd_contravariant_projection_2d": [c_void_p] * 2 + [c_int] * 3 + [c_void_p],
+    "selfamd_update_grk": [c_void_p] * 3 + [c_double] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_calculate_dsdt": [c_void_p] * 3 + [c_int] * 3 + [c_void_p],
+    "selfamd_semquad_generate": [c_int] * 4 + [c_void_p] * 11 + [c_void_p],
+    "selfamd_sw2d_velocity": [c_void_p] * 4 + [c_int] * 2 + [c_void_p],
+    "selfamd_sw2d_set_boundary_condition": [c_void_p] * 5 + [c_int] * 2 + [c_void_p],
+    "selfamd_sw2d_source": [c_void_p] * 3 + [c_double] * 2 + [c_int] * 2 + [c_void_p],
+    "selfamd_sw2d_flux": [c_void_p] * 3 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_sw2d_riemann": [c_void_p] * 7 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_lsw2d_set_boundary_condition": [c_void_p] * 4 + [c_int] * 2 + [c_void_p],
+    "selfamd_lsw2d_source": [c_void_p] * 3 + [c_int] * 2 + [c_void_p],
+    "selfamd_lsw2d_flux": [c_void_p] * 3 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_lsw2d_riemann": [c_void_p] * 6 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_diagnostics": [c_void_p] * 2 + [c_double] * 2 + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_primitive": [c_void_p] * 2 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_entropy_vars": [c_void_p] * 2 + [c_double] + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_set_boundary_condition": [c_void_p] * 11 + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_source": [c_void_p] * 4 + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_flux": [c_void_p] * 4 + [c_int] * 2 + [c_void_p],
+    "selfamd_euler2d_riemann": [c_void_p] * 9 + [c_int] * 2 + [c_void_p],
+    "selfamd_model2d_nvar": [c_int],
+    "selfamd_model2d_create": [c_void_p, c_int, c_int, c_int] + [c_void_p] * 5 + [c_void_p],
+    "selfamd_model2d_destroy": [c_void_p],
+    "selfamd_model2d_set_param": [c_void_p, c_int, c_double],
+    "selfamd_model2d_set_field": [c_void_p, c_int, c_void_p],
+    "selfamd_model2d_set_solution": [c_void_p, c_void_p],
+    "selfamd_model2d_get_solution": [c_void_p, c_void_p],
+    "selfamd_model2d_tendency": [c_void_p, c_void_p],
+    "selfamd_model2d_forward_step": [c_void_p, c_int, c_double],
+    "selfamd_model2d_synchronize": [c_void_p],
+    "selfamd_model2d_time_steps": [c_void_p, c_int, c_double, c_void_p],
+    "selfamd_model2d_get_time": [c_void_p, c_void_p],
+    "selfamd_model2d_set_time": [c_void_p, c_double],
+}
+
+
+class SelfAmdError(RuntimeError):
+    """A C entry point returned a non-zero status."""
+
+
+_lib = None
+
+
+def load():
+    """Load libself_amd.so once; raise ImportError (no fallback) when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"self_amd native library not found at {LIB_PATH}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C self_amd/csrc`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_char_p if name == "selfamd_last_error" else c_int
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call an entry point and raise SelfAmdError with the library's message on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.selfamd_last_error().decode(errors="replace")
+        raise SelfAmdError(f"{name} failed (status {rc}): {msg}")
+    return rc

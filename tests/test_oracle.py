@@ -1,0 +1,308 @@
+"""Pins the CPU oracle against closed forms, invariants and the reference's mesh fixtures.
+
+Every oracle function used as a parity reference is checked here by something other than a
+re-typed copy of its own formula: textbook quadrature, polynomial exactness, Gauss SBP
+identities, affine-map geometry, brute-force permutations, conservation laws, free-stream
+preservation and temporal convergence order.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.cases import Case, rel_err
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- quadrature (a1)
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 9, 12])
+def test_gauss_nodes_match_leggauss(N):
+    x, w = oracle.legendre_gauss(N)
+    xr, wr = np.polynomial.legendre.leggauss(N + 1)
+    order = np.argsort(x)
+    np.testing.assert_allclose(x[order], xr, atol=2e-15)
+    np.testing.assert_allclose(w[order], wr, atol=2e-15)
+    assert abs(w.sum() - 2.0) < 1e-14
+
+
+def test_gauss_n3_node_order_quirk():
+    # Newton from the j=2 seed jumps across the origin (src/SELF_Quadrature.f90:227-243), so
+    # the reference's N=3 nodes come out as (-x1, +x0, -x0, +x1) with x0 < x1.
+    x, _ = oracle.legendre_gauss(3)
+    a, b = np.sqrt(3 / 7 - 2 / 7 * np.sqrt(6 / 5)), np.sqrt(3 / 7 + 2 / 7 * np.sqrt(6 / 5))
+    np.testing.assert_allclose(x, [-b, a, -a, b], atol=1e-15)
+    for N in (5, 7):
+        xn, _ = oracle.legendre_gauss(N)
+        assert np.all(np.diff(xn) > 0)
+
+
+@pytest.mark.parametrize("N,expect", [(2, [-1, 0, 1]), (4, [-1, -np.sqrt(3 / 7), 0, np.sqrt(3 / 7), 1])])
+def test_gauss_lobatto_closed_form(N, expect):
+    x, w = oracle.legendre_gauss_lobatto(N)
+    np.testing.assert_allclose(x, expect, atol=1e-15)
+    for k in range(2 * N):  # exact for degree <= 2N-1
+        assert abs(np.dot(w, x ** k) - (2.0 / (k + 1) if k % 2 == 0 else 0.0)) < 1e-14
+
+
+# ---------------------------------------------------------------- Lagrange matrices (a1)
+@pytest.mark.parametrize("N", [3, 5, 7])
+def test_lagrange_identities(N):
+    L = oracle.lagrange_init(N, oracle.GAUSS, 12, oracle.UNIFORM)
+    x, w = L["controlPoints"], L["qWeights"]
+    D = L["dMatrix"]              # D[i, ii] = Fortran dMatrix(ii, i) = l'_ii(x_i)
+    dg = L["dgMatrix"]            # dg[i, ii] = Fortran dgMatrix(ii, i)
+    I = L["iMatrix"]              # I[m, ii] = l_ii(t_m)
+    t = L["targetPoints"]
+    b0, b1 = L["bMatrix"]
+    np.testing.assert_allclose(t, np.linspace(-1, 1, 13), atol=1e-15)
+    for k in range(N + 1):
+        f = x ** k
+        np.testing.assert_allclose(I @ f, t ** k, atol=1e-13)                          # interpolation
+        np.testing.assert_allclose(D @ f, k * x ** max(k - 1, 0) if k else 0 * x, atol=1e-12)  # derivative
+        assert abs(b0 @ f - (-1.0) ** k) < 1e-13 and abs(b1 @ f - 1.0) < 1e-13          # boundary values
+        # weak derivative + boundary terms = strong derivative (Gauss SBP)
+        weak = dg @ f + (b1 * 1.0 - b0 * (-1.0) ** k) / w
+        np.testing.assert_allclose(weak, k * x ** max(k - 1, 0) if k else 0 * x, atol=1e-11)
+    np.testing.assert_allclose(I.sum(axis=1), 1.0, atol=1e-14)
+
+
+# ---------------------------------------------------------------- 2D operators (a2, a3, a12)
+def _poly(N, nEl=3, nVar=2, seed=0):
+    """Random tensor polynomials of degree <= N per element/variable, on Gauss nodes."""
+    rng = np.random.default_rng(seed)
+    x, _ = oracle.legendre_gauss(N)
+    C = rng.normal(size=(nEl, nVar, N + 1, N + 1))  # coefficients c[a, b] x^a y^b
+    def ev(xx, yy, Cc):
+        return sum(Cc[a, b] * xx ** a * yy ** b for a in range(N + 1) for b in range(N + 1))
+    X, Y = np.meshgrid(x, x)  # X[j, i] = x_i, Y[j, i] = x_j
+    f = np.stack([[ev(X, Y, C[e, v]) for v in range(nVar)] for e in range(nEl)])
+    return x, C, ev, f
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_boundary_interp_exact(N):
+    x, C, ev, f = _poly(N)
+    bM = oracle.lagrange_init(N, oracle.GAUSS, N, oracle.GAUSS)["bMatrix"]
+    fb = oracle.scalar_boundary_interp(N, bM, np.ascontiguousarray(f))
+    for e in range(f.shape[0]):
+        for v in range(f.shape[1]):
+            np.testing.assert_allclose(fb[e, 0, v], ev(x, -1.0, C[e, v]), atol=1e-10)  # south y=-1
+            np.testing.assert_allclose(fb[e, 1, v], ev(1.0, x, C[e, v]), atol=1e-10)   # east x=1
+            np.testing.assert_allclose(fb[e, 2, v], ev(x, 1.0, C[e, v]), atol=1e-10)   # north
+            np.testing.assert_allclose(fb[e, 3, v], ev(-1.0, x, C[e, v]), atol=1e-10)  # west
+    vec = np.ascontiguousarray(np.stack([f, 2 * f], axis=-1))
+    vb = oracle.vector_boundary_interp(N, bM, vec)
+    np.testing.assert_allclose(vb[..., 0], fb, atol=1e-12)
+    np.testing.assert_allclose(vb[..., 1], 2 * fb, atol=1e-12)
+
+
+def test_grid_interp_exact():
+    N, M = 5, 9
+    x, C, ev, f = _poly(N)
+    L = oracle.lagrange_init(N, oracle.GAUSS, M, oracle.UNIFORM)
+    g = oracle.scalar_grid_interp(N, M, L["iMatrix"], np.ascontiguousarray(f))
+    t = L["targetPoints"]
+    T, S = np.meshgrid(t, t)
+    for e in range(f.shape[0]):
+        np.testing.assert_allclose(g[e, 0], ev(T, S, C[e, 0]), atol=1e-10)
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_gradient_and_weak_divergence_exact(N):
+    x, C, ev, f = _poly(N, nVar=2)
+    L = oracle.lagrange_init(N, oracle.GAUSS, N, oracle.GAUSS)
+    X, Y = np.meshgrid(x, x)
+    F = np.ascontiguousarray(np.stack([f[:, :1], f[:, 1:]], axis=-1))  # one vector variable
+    grad = oracle.vector_gradient(N, L["dMatrix"], F)  # [e, v, j, i, c, r] = d F_r / d xi_c
+    def dpoly(Cc, dx):
+        return lambda xx, yy: sum(Cc[a, b] * (a * xx ** max(a - 1, 0) * yy ** b if dx else b * xx ** a * yy ** max(b - 1, 0))
+                                  for a in range(N + 1) for b in range(N + 1))
+    for e in range(f.shape[0]):
+        np.testing.assert_allclose(grad[e, 0, :, :, 0, 0], dpoly(C[e, 0], True)(X, Y), atol=1e-9)
+        np.testing.assert_allclose(grad[e, 0, :, :, 1, 1], dpoly(C[e, 1], False)(X, Y), atol=1e-9)
+    strong = oracle.vector_divergence(N, L["dMatrix"], F)
+    # outward computational normal fluxes: E = F1(1,.), W = -F1(-1,.), N = F2(.,1), S = -F2(.,-1)
+    bF = np.zeros((f.shape[0], 4, 1, N + 1))
+    for e in range(f.shape[0]):
+        bF[e, 0, 0] = -ev(x, -1.0, C[e, 1])
+        bF[e, 1, 0] = ev(1.0, x, C[e, 0])
+        bF[e, 2, 0] = ev(x, 1.0, C[e, 1])
+        bF[e, 3, 0] = -ev(-1.0, x, C[e, 0])
+    weak = oracle.vector_dg_divergence(N, L["dgMatrix"], L["bMatrix"], L["qWeights"], F, bF)
+    for e in range(f.shape[0]):
+        exact = dpoly(C[e, 0], True)(X, Y) + dpoly(C[e, 1], False)(X, Y)
+        np.testing.assert_allclose(strong[e, 0], exact, atol=1e-9)
+        np.testing.assert_allclose(weak[e, 0], exact, atol=1e-9)
+
+
+# ---------------------------------------------------------------- mesh + geometry (a16, a18)
+def _structured(nx, ny, periodic=False):
+    import self_amd
+    return self_amd.Mesh2D.structured(nx, ny, 0.0, 2.0, 0.0, 1.0, periodic=(periodic, periodic))
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_affine_geometry_closed_form(N):
+    mesh = _structured(4, 2)
+    g = oracle.Geometry(N, mesh.nodeCoords, 1)
+    dx, dy = 0.5, 0.5
+    np.testing.assert_allclose(g.J, dx * dy / 4, rtol=1e-13)
+    np.testing.assert_allclose(g.dxds[..., 0, 0], dx / 2, rtol=1e-13)   # dxds(1,1)
+    np.testing.assert_allclose(g.dxds[..., 1, 1], dy / 2, rtol=1e-13)   # dxds(2,2)
+    np.testing.assert_allclose(g.dxds[..., 1, 0], 0.0, atol=1e-14)
+    normals = {0: (0, -1), 1: (1, 0), 2: (0, 1), 3: (-1, 0)}
+    for s, n in normals.items():
+        np.testing.assert_allclose(g.nHatb[:, s], np.broadcast_to(n, g.nHatb[:, s].shape), atol=1e-13)
+        np.testing.assert_allclose(g.nScaleb[:, s], dx / 2 if s in (0, 2) else dy / 2, rtol=1e-13)
+    x, _ = oracle.legendre_gauss(N)
+    e = 5  # ix = 1, iy = 1
+    np.testing.assert_allclose(g.x[e, 0, :, 0], 0.5 + (x + 1) * dx / 2, atol=1e-15)
+    assert abs(g.covariant_arc_min() - 2 * min(dx, dy) / 2 / N ** 2) < 1e-14
+
+
+def _load_hopr(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        fx = json.load(f)
+    si3 = np.array(fx["SideInfo"]["data"], np.int32).reshape(fx["SideInfo"]["shape"])
+    nc3 = np.array(fx["NodeCoords"]["data"]).reshape(fx["NodeCoords"]["shape"])
+    gn3 = np.array(fx["GlobalNodeIDs"]["data"], np.int32)
+    return fx, si3, nc3, gn3
+
+
+@pytest.mark.parametrize("name", ["hopr_block2d", "hopr_geophysical_block2d_medium", "hopr_circle"])
+def test_hopr_connectivity_geometric_consistency(name):
+    fx, si3, nc3, gn3 = _load_hopr(name)
+    nGeo = fx["Ngeo"]
+    si, nc, gn = oracle.hopr_to_mesh2d(nGeo, si3, nc3, gn3)
+    nEl = fx["nElems"]
+    assert si.shape == (nEl, 4, 5)
+    corner = {0: [(0, 0), (nGeo, 0)], 1: [(nGeo, 0), (nGeo, nGeo)], 2: [(0, nGeo), (nGeo, nGeo)],
+              3: [(0, 0), (0, nGeo)]}  # CGNS side -> ordered corners (i, j)
+    nint = 0
+    for e in range(nEl):
+        for s in range(4):
+            e2, code, bc = si[e, s, 2], si[e, s, 3], si[e, s, 4]
+            if bc != 0:
+                assert e2 == 0 and code == -10  # boundary sides keep HOPr's 0 minus 10
+                continue
+            nint += 1
+            s2, flip = code // 10 - 1, code % 10
+            # neighbour symmetry
+            assert si[e2 - 1, s2, 2] == e + 1 and si[e2 - 1, s2, 3] // 10 - 1 == s
+            assert si[e2 - 1, s2, 3] % 10 == flip
+            # the two sides share their end points, in the same order iff flip == 0
+            p1 = [nc[e, j, i] for (i, j) in corner[s]]
+            p2 = [nc[e2 - 1, j, i] for (i, j) in corner[s2]]
+            if flip == 0:
+                np.testing.assert_allclose(p1, p2, atol=1e-12)
+            else:
+                np.testing.assert_allclose(p1, p2[::-1], atol=1e-12)
+    assert nint > 0
+
+
+def test_structured_mesh_flip_recalc_and_symmetry():
+    for periodic in (False, True):
+        m = _structured(5, 3, periodic)
+        si = oracle.recalculate_flip(m.sideInfo, m.globalNodeIDs, 1)
+        np.testing.assert_array_equal(si, m.sideInfo)
+        inter = m.sideInfo[..., 4] == 0
+        assert np.all(m.sideInfo[..., 3][inter] % 10 == 0)
+        if periodic:
+            assert inter.all()
+
+
+def test_side_exchange_permutation():
+    N = 4
+    m = _structured(3, 3, True)
+    rng = np.random.default_rng(3)
+    b = rng.normal(size=(9, 4, 2, N + 1))
+    ext = oracle.side_exchange_scalar(N, m.sideInfo, b)
+    for e in range(9):
+        for s in range(4):
+            e2, s2, flip = m.sideInfo[e, s, 2] - 1, m.sideInfo[e, s, 3] // 10 - 1, m.sideInfo[e, s, 3] % 10
+            want = b[e2, s2][:, ::-1] if flip else b[e2, s2]
+            np.testing.assert_array_equal(ext[e, s], want)
+
+
+def test_gradient_sf_affine_exact():
+    N = 5
+    m = _structured(2, 2)
+    g = oracle.Geometry(N, m.nodeCoords, 1)
+    L = oracle.lagrange_init(N, oracle.GAUSS, N, oracle.GAUSS)
+    X, Y = g.x[..., 0], g.x[..., 1]
+    f = np.ascontiguousarray((X ** 3 * Y - 2 * Y ** 2)[:, None])
+    grad = oracle.gradient_sf(N, L["dMatrix"], g.dsdx, g.J, f)
+    np.testing.assert_allclose(grad[:, 0, ..., 0], 3 * X ** 2 * Y, atol=1e-11)
+    np.testing.assert_allclose(grad[:, 0, ..., 1], X ** 3 - 4 * Y, atol=1e-11)
+
+
+# ---------------------------------------------------------------- tendency invariants (a13)
+def test_sw_mass_conservation_periodic():
+    c = Case("sw", 5, 4, 4, periodic=True)
+    d = c.om.tendency()
+    w = c.weights
+    mass_rate = np.sum(d[:, 2] * c.geom.J * w[None, :, None] * w[None, None, :])
+    assert abs(mass_rate) < 1e-14 * np.abs(d[:, 2]).max()
+    m0 = c.om.mass(2)
+    c.om.rk3_steps(5, c.dt())
+    assert abs(c.om.mass(2) - m0) < 1e-14 * m0
+
+
+def test_sw_quiet_fluid():
+    # test/ShallowWater2D_QuietFluid.f90: flat topography, H = 1 at rest stays at rest
+    c = Case("sw", 7, 3, 3, bcid=oracle.BC_RADIATION)
+    nel = c.mesh.nElem
+    c.fields["topo"] = np.ones((nel, 8, 8))
+    sol = np.zeros((nel, 3, 8, 8))
+    sol[:, 2] = 1.0
+    c.sol = sol
+    c.apply_oracle(c.om)
+    c.om.rk3_steps(20, 1e-3)
+    s = c.om.solution
+    assert np.abs(s[:, :2]).max() <= 1000 * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("kind", ["lsw", "euler"])
+def test_free_stream_preservation(kind):
+    c = Case(kind, 4, 3, 3, periodic=True)
+    nel = c.mesh.nElem
+    sol = np.zeros((nel, c.om.nVar, 5, 5))
+    if kind == "lsw":
+        sol[:, 0], sol[:, 1], sol[:, 2] = 0.3, -0.2, 0.1
+    else:
+        c.fields["phi"] = np.zeros((nel, 5, 5))
+        sol[:, 0], sol[:, 1], sol[:, 2] = 0.3, -0.2, 1.0
+        sol[:, 3] = 1.0 / 0.4 + 0.5 * (0.09 + 0.04)
+    c.sol = sol
+    c.apply_oracle(c.om)
+    assert np.abs(c.om.tendency()).max() < 1e-12
+
+
+def test_rk3_third_order():
+    # temporal convergence of the full semi-discrete LSW system to a fixed time
+    c = Case("lsw", 3, 4, 4, periodic=True)
+    T = 0.05
+    sols = []
+    for n in (4, 8, 16, 64):
+        om = oracle.Model(oracle.LSW, 3, c.mesh.sideInfo, c.geom)
+        c.apply_oracle(om)
+        om.rk3_steps(n, T / n)
+        sols.append(om.solution)
+    ref = sols[-1]
+    e = [np.abs(s - ref).max() for s in sols[:3]]
+    rates = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    assert np.all(rates > 2.7), rates
+
+
+def test_threads_bit_identical():
+    c = Case("sw", 7, 4, 4, periodic=True)
+    oracle.set_threads(1)
+    a = c.om.tendency()
+    oracle.set_threads(4)
+    b = c.om.tendency()
+    oracle.set_threads(1)
+    np.testing.assert_array_equal(a, b)
+    assert rel_err(a, b).max() == 0.0
