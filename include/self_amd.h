@@ -20,6 +20,7 @@
  */
 #ifndef SELF_AMD_H
 #define SELF_AMD_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -53,6 +54,20 @@ extern "C" {
 
 const char *selfamd_last_error(void);
 int selfamd_version(void);
+
+/* ------------------------------------------------------------------------------------ */
+/* Device plumbing: hosts without a HIP runtime of their own (ctypes, iso_c_binding)      */
+/* allocate and copy through these, so a process loads only the runtime this library     */
+/* links.  No reference counterpart (the reference uses hipfort, src/SELF_Memory.f90).    */
+/* ------------------------------------------------------------------------------------ */
+int selfamd_device_count(int *count);
+int selfamd_device_name(int device, char *name, int len);
+int selfamd_malloc(void **ptr, size_t bytes);
+int selfamd_free(void *ptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device (synchronous) */
+int selfamd_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int selfamd_memset(void *ptr, int value, size_t bytes);
+int selfamd_device_synchronize(void);
 
 /* ------------------------------------------------------------------------------------ */
 /* Host setup                                                                            */

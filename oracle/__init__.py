@@ -201,7 +201,8 @@ def contravariant_projection(N, dsdx, f):
 # ---- mesh ----
 def recalculate_flip(sideInfo, globalNodeIDs, nGeo):
     si = np.ascontiguousarray(sideInfo, np.int32).copy()
-    lib().or_recalculate_flip(si.shape[0], nGeo, P(si), P(np.ascontiguousarray(globalNodeIDs, np.int32)))
+    gn = np.ascontiguousarray(globalNodeIDs, np.int32)  # named: must outlive the call
+    lib().or_recalculate_flip(si.shape[0], nGeo, P(si), P(gn))
     return si
 
 
@@ -211,8 +212,9 @@ def hopr_to_mesh2d(nGeo, sideInfo3D, nodeCoords3D, globalNodeIDs3D):
     si = np.zeros((nEl, 4, 5), np.int32)
     nc = np.zeros((nEl, G, G, 2))
     gn = np.zeros((nEl, G, G), np.int32)
-    lib().or_hopr_to_mesh2d(nEl, nGeo, P(si3), P(np.ascontiguousarray(nodeCoords3D, np.float64)),
-                            P(np.ascontiguousarray(globalNodeIDs3D, np.int32)), P(si), P(nc), P(gn))
+    nc3 = np.ascontiguousarray(nodeCoords3D, np.float64)  # named: must outlive the call
+    gn3 = np.ascontiguousarray(globalNodeIDs3D, np.int32)
+    lib().or_hopr_to_mesh2d(nEl, nGeo, P(si3), P(nc3), P(gn3), P(si), P(nc), P(gn))
     return si, nc, gn
 
 
@@ -285,19 +287,24 @@ class Model:
         lib().or_model_set_solution(self._p, P(a))
 
     def set_topography(self, b):
-        lib().or_sw_set_topography(self._p, P(self._scal(b)))
+        a = self._scal(b)  # named: must outlive the call
+        lib().or_sw_set_topography(self._p, P(a))
 
     def set_bathymetry(self, H):
-        lib().or_lsw_set_bathymetry(self._p, P(self._scal(H)))
+        a = self._scal(H)  # named: must outlive the call
+        lib().or_lsw_set_bathymetry(self._p, P(a))
 
     def set_coriolis(self, f):
-        lib().or_lsw_set_coriolis(self._p, P(self._scal(f)))
+        a = self._scal(f)  # named: must outlive the call
+        lib().or_lsw_set_coriolis(self._p, P(a))
 
     def set_gravity(self, phi):
-        lib().or_euler_set_gravity(self._p, P(self._scal(phi)))
+        a = self._scal(phi)  # named: must outlive the call
+        lib().or_euler_set_gravity(self._p, P(a))
 
     def set_drag(self, cd):
-        lib().or_euler_set_drag(self._p, P(self._scal(cd)))
+        a = self._scal(cd)  # named: must outlive the call
+        lib().or_euler_set_drag(self._p, P(a))
 
     def set_prescribed(self):
         lib().or_euler_set_prescribed(self._p)

@@ -2,9 +2,10 @@
 
 Host-side mirror of the SELF model API for the 2D DGSEM path (Lagrange, Mesh2D, SEMQuad,
 Model2D and the LinearShallowWater / ShallowWater / CompressibleIdealGas2D plug-ins).  All
-numerics run in libself_amd.so (hand-written HIP kernels for gfx950) through its C ABI;
-PyTorch only provides device memory.  There is no CPU fallback: without the native
-library or a GPU the compute calls raise.
+numerics run in libself_amd.so (hand-written HIP kernels for gfx950) through its C ABI.
+Device memory is allocated through the library's own C ABI (DeviceArray), so the process
+loads only the HIP runtime libself_amd.so links.  There is no CPU fallback: without the
+native library or a GPU the compute calls raise.
 
 Array convention: a Fortran array a(0:N,0:N,nVar,nEl) is held in C order with the shape
 reversed, a[e, v, j, i]; boundary(0:N,nVar,4,nEl) is b[e, s, v, i]; sideInfo(5,4,nEl) is
@@ -23,39 +24,99 @@ MODEL_LSW, MODEL_SW, MODEL_EULER = 1, 2, 3
 PARAM_G, PARAM_FCORI, PARAM_GAMMA, PARAM_RGAS = 1, 2, 3, 4
 FIELD_TOPOGRAPHY, FIELD_BATHYMETRY, FIELD_CORIOLIS, FIELD_POTENTIAL, FIELD_DRAG = 1, 2, 3, 4, 5
 
-__all__ = ["Lagrange", "Mesh2D", "SEMQuad", "Model2D", "ShallowWater", "LinearShallowWater",
-           "CompressibleIdealGas2D", "SelfAmdError", "load", "GAUSS", "GAUSS_LOBATTO", "UNIFORM",
+__all__ = ["DeviceArray", "device_count", "device_name", "synchronize", "Lagrange", "Mesh2D", "SEMQuad",
+           "Model2D", "ShallowWater", "LinearShallowWater", "CompressibleIdealGas2D", "SelfAmdError", "load", "GAUSS", "GAUSS_LOBATTO", "UNIFORM",
            "BC_PRESCRIBED", "BC_RADIATION", "BC_NONORMALFLOW"]
 
 
+class DeviceArray:
+    """A device allocation owned by Python (hipMalloc through the library's C ABI).
+
+    Holds `shape`/`dtype` metadata for a C-order array; `ptr` is the device address passed
+    to entry points.  Copies are synchronous."""
+
+    def __init__(self, shape, dtype=np.float64, zero=False):
+        self.shape = tuple(int(n) for n in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        _require_device()
+        call("selfamd_malloc", ctypes.byref(self.ptr), self.nbytes)
+        if zero:
+            call("selfamd_memset", self.ptr, 0, self.nbytes)
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or (a.dtype if isinstance(a, np.ndarray) else np.float64))
+        d = cls(a.shape, a.dtype)
+        d.copy_from_host(a)
+        return d
+
+    def copy_from_host(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        call("selfamd_memcpy", self.ptr, ctypes.c_void_p(a.ctypes.data), self.nbytes, 1)
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        call("selfamd_memcpy", ctypes.c_void_p(out.ctypes.data), self.ptr, self.nbytes, 2)
+        return out
+
+    def reshape(self, *shape):
+        """View with another shape (same allocation)."""
+        shape = shape[0] if len(shape) == 1 and isinstance(shape[0], tuple) else shape
+        v = object.__new__(DeviceArray)
+        v.shape, v.dtype, v.nbytes, v.ptr = tuple(shape), self.dtype, self.nbytes, self.ptr
+        v._base = self
+        assert int(np.prod(v.shape)) * v.dtype.itemsize == self.nbytes
+        return v
+
+    def __del__(self):
+        if getattr(self, "_base", None) is None and getattr(self, "ptr", None) is not None and self.ptr.value:
+            try:
+                load().selfamd_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    call("selfamd_device_count", ctypes.byref(n))
+    return n.value
+
+
+def device_name(device=0):
+    buf = ctypes.create_string_buffer(256)
+    call("selfamd_device_name", device, buf, 256)
+    return buf.value.decode()
+
+
+def synchronize():
+    call("selfamd_device_synchronize")
+
+
+def _require_device():
+    if device_count() < 1:
+        raise SelfAmdError("self_amd compute calls need a ROCm GPU (no HIP device visible)")
+
+
 def _ptr(a):
-    """Address of a numpy array (host) or torch tensor (device) for the C ABI."""
-    if isinstance(a, np.ndarray):
-        assert a.flags["C_CONTIGUOUS"]
-        return ctypes.c_void_p(a.ctypes.data)
-    assert a.is_contiguous()
-    return ctypes.c_void_p(a.data_ptr())
+    """Address for the C ABI: numpy arrays (host) or DeviceArray (device)."""
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
 
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
-        raise SelfAmdError("self_amd compute calls need a ROCm GPU (torch.cuda.is_available() is False)")
-    return torch
-
-
-def _dev(x, dtype=None):
-    """Contiguous float64 (or given dtype) device tensor from numpy or torch input."""
-    torch = _torch()
-    dtype = dtype or torch.float64
-    if isinstance(x, np.ndarray):
-        return torch.from_numpy(np.ascontiguousarray(x)).to(device="cuda", dtype=dtype)
-    return x.to(device="cuda", dtype=dtype).contiguous()
-
-
-def _stream():
-    torch = _torch()
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+def _dev(x, shape=None):
+    """DeviceArray from numpy input (copied) or a DeviceArray (returned as is)."""
+    if isinstance(x, DeviceArray):
+        return x if shape is None else x.reshape(shape)
+    a = np.ascontiguousarray(np.asarray(x, np.float64))
+    if shape is not None:
+        a = a.reshape(shape)
+    return DeviceArray.from_host(a)
 
 
 class Lagrange:
@@ -85,9 +146,10 @@ class Lagrange:
         self._dev = None
 
     def device(self):
-        """Device copies (iMatrix, dMatrix, dgMatrix, bMatrix, qWeights) as torch tensors."""
+        """Device copies (iMatrix, dMatrix, dgMatrix, bMatrix, qWeights) as DeviceArrays."""
         if self._dev is None:
-            self._dev = {k: _dev(getattr(self, k)) for k in ("iMatrix", "dMatrix", "dgMatrix", "bMatrix", "qWeights")}
+            self._dev = {k: DeviceArray.from_host(getattr(self, k))
+                         for k in ("iMatrix", "dMatrix", "dgMatrix", "bMatrix", "qWeights")}
         return self._dev
 
 
@@ -123,8 +185,9 @@ class Mesh2D:
         si = np.zeros((nEl, 4, 5), np.int32)
         nc = np.zeros((nEl, G, G, 2))
         gn = np.zeros((nEl, G, G), np.int32)
-        call("selfamd_mesh2d_from_hopr", nEl, nGeo, _ptr(si3), _ptr(np.ascontiguousarray(nodeCoords3D, np.float64)),
-             _ptr(np.ascontiguousarray(globalNodeIDs3D, np.int32)), _ptr(si), _ptr(nc), _ptr(gn))
+        nc3 = np.ascontiguousarray(nodeCoords3D, np.float64)  # named: must outlive the call
+        gn3 = np.ascontiguousarray(globalNodeIDs3D, np.int32)
+        call("selfamd_mesh2d_from_hopr", nEl, nGeo, _ptr(si3), _ptr(nc3), _ptr(gn3), _ptr(si), _ptr(nc), _ptr(gn))
         return cls(si, nc, gn, nGeo)
 
     def RecalculateFlip(self):
@@ -136,34 +199,31 @@ class Mesh2D:
 
 class SEMQuad:
     """Geometry (metric terms) generated on the device (GenerateFromMesh_SEMQuad,
-    src/SELF_Geometry.f90:393-552).  Attributes are device tensors in the C-order views:
+    src/SELF_Geometry.f90:393-552).  Attributes are DeviceArrays in the C-order views:
     x[e,j,i,2], xb[e,s,i,2], dxds/dsdx[e,j,i,2(c),2(r)], dsdxb[e,s,i,2,2], nHatb[e,s,i,2],
     nScaleb[e,s,i], J[e,j,i], Jb[e,s,i]."""
 
     def __init__(self, interp, mesh):
-        torch = _torch()
         N, nEl, P = interp.N, mesh.nElem, interp.N + 1
         self.interp, self.nElem = interp, nEl
-        f64 = dict(dtype=torch.float64, device="cuda")
-        self.x = torch.empty((nEl, P, P, 2), **f64)
-        self.xb = torch.empty((nEl, 4, P, 2), **f64)
-        self.dxds = torch.empty((nEl, P, P, 2, 2), **f64)
-        self.dxdsb = torch.empty((nEl, 4, P, 2, 2), **f64)
-        self.dsdx = torch.empty((nEl, P, P, 2, 2), **f64)
-        self.dsdxb = torch.empty((nEl, 4, P, 2, 2), **f64)
-        self.nHatb = torch.empty((nEl, 4, P, 2), **f64)
-        self.nScaleb = torch.empty((nEl, 4, P), **f64)
-        self.J = torch.empty((nEl, P, P), **f64)
-        self.Jb = torch.empty((nEl, 4, P), **f64)
-        nc = _dev(mesh.nodeCoords)
-        torch.cuda.synchronize()
+        self.x = DeviceArray((nEl, P, P, 2))
+        self.xb = DeviceArray((nEl, 4, P, 2))
+        self.dxds = DeviceArray((nEl, P, P, 2, 2))
+        self.dxdsb = DeviceArray((nEl, 4, P, 2, 2))
+        self.dsdx = DeviceArray((nEl, P, P, 2, 2))
+        self.dsdxb = DeviceArray((nEl, 4, P, 2, 2))
+        self.nHatb = DeviceArray((nEl, 4, P, 2))
+        self.nScaleb = DeviceArray((nEl, 4, P))
+        self.J = DeviceArray((nEl, P, P))
+        self.Jb = DeviceArray((nEl, 4, P))
+        nc = DeviceArray.from_host(mesh.nodeCoords)
         call("selfamd_semquad_generate", N, interp.controlNodeType, mesh.nGeo, nEl, _ptr(nc),
              *[_ptr(t) for t in (self.x, self.xb, self.dxds, self.dxdsb, self.dsdx, self.dsdxb, self.nHatb,
-                                 self.nScaleb, self.J, self.Jb)], _stream())
+                                 self.nScaleb, self.J, self.Jb)], None)
 
     def CovariantArcMin(self):
         """CovariantArcMin_SEMQuad (src/SELF_Geometry.f90:554-582)."""
-        d = self.dxds.cpu().numpy()  # [e, j, i, c, r] = dxds(r, c)
+        d = self.dxds.to_host()  # [e, j, i, c, r] = dxds(r, c)
         dx = np.sqrt(d[..., 0, 0] ** 2 + d[..., 1, 0] ** 2)  # sqrt(dxds(1,1)^2 + dxds(1,2)^2)
         dy = np.sqrt(d[..., 0, 1] ** 2 + d[..., 1, 1] ** 2)  # sqrt(dxds(2,1)^2 + dxds(2,2)^2)
         N = self.interp.N
@@ -177,15 +237,14 @@ class Model2D:
     kind = None
 
     def __init__(self, mesh, geometry):
-        torch = _torch()
         self.mesh, self.geometry = mesh, geometry
         self.N = geometry.interp.N
         self.nElem = mesh.nElem
         self.nVar = load().selfamd_model2d_nvar(self.kind)
         self._h = ctypes.c_void_p()
-        torch.cuda.synchronize()
         call("selfamd_model2d_create", ctypes.byref(self._h), self.kind, self.N, self.nElem, _ptr(mesh.sideInfo),
              _ptr(geometry.dsdx), _ptr(geometry.J), _ptr(geometry.nHatb), _ptr(geometry.nScaleb), None)
+        self._buf = DeviceArray(self.shape)
         self.dt = 0.0
 
     def __del__(self):
@@ -206,31 +265,24 @@ class Model2D:
         call("selfamd_model2d_set_param", self._h, which, float(value))
 
     def _field(self, which, values):
-        torch = _torch()
-        f = _dev(values).reshape(self.nElem, self.N + 1, self.N + 1).contiguous()
-        torch.cuda.synchronize()
+        f = _dev(values, (self.nElem, self.N + 1, self.N + 1))
         call("selfamd_model2d_set_field", self._h, which, _ptr(f))
 
     def SetSolution(self, interior):
-        torch = _torch()
-        s = _dev(interior).reshape(self.shape).contiguous()
-        torch.cuda.synchronize()
+        """SetSolution (src/SELF_Model2D.f90:313-332) from interior[e, v, j, i] (numpy or DeviceArray)."""
+        s = _dev(interior, self.shape)
         call("selfamd_model2d_set_solution", self._h, _ptr(s))
 
     @property
     def solution(self):
-        torch = _torch()
-        out = torch.empty(self.shape, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        call("selfamd_model2d_get_solution", self._h, _ptr(out))
-        return out
+        """Host copy of the solution interior[e, v, j, i]."""
+        call("selfamd_model2d_get_solution", self._h, _ptr(self._buf))
+        return self._buf.to_host()
 
     def CalculateTendency(self):
-        torch = _torch()
-        out = torch.empty(self.shape, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        call("selfamd_model2d_tendency", self._h, _ptr(out))
-        return out
+        """CalculateTendency (src/SELF_Model2D.f90:816-859); returns a host copy of dSdt[e, v, j, i]."""
+        call("selfamd_model2d_tendency", self._h, _ptr(self._buf))
+        return self._buf.to_host()
 
     def ForwardStep(self, nSteps=1, dt=None):
         """nSteps low-storage RK3 steps (LowStorageRK3_timeIntegrator, src/SELF_Model.f90:704-732)."""

@@ -16,8 +16,17 @@ c_dp = ctypes.POINTER(ctypes.c_double)
 
 # (name, argtypes) for every exported entry point; the header is the source of truth and
 # tests/test_abi.py checks that each declared symbol is listed and exported.
+c_size_t = ctypes.c_size_t
+
 SIGNATURES = {
     "selfamd_last_error": [],
+    "selfamd_device_count": [c_void_p],
+    "selfamd_device_name": [c_int, ctypes.c_char_p, c_int],
+    "selfamd_malloc": [c_void_p, c_size_t],
+    "selfamd_free": [c_void_p],
+    "selfamd_memcpy": [c_void_p, c_void_p, c_size_t, c_int],
+    "selfamd_memset": [c_void_p, c_int, c_size_t],
+    "selfamd_device_synchronize": [],
     "selfamd_version": [],
     "selfamd_lagrange_init": [c_int, c_int, c_int, c_int] + [c_void_p] * 8,
     "selfamd_mesh2d_structured": [c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int,

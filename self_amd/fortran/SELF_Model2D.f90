@@ -1,0 +1,245 @@
+! SELF_Model2D.f90 -- Model2D (src/SELF_Model2D.f90:18-71) and the three physics plug-ins
+! (src/models/SELF_LinearShallowWater.f90, SELF_ShallowWater.f90, SELF_CompressibleIdealGas2D.f90)
+! for the self_amd engine.
+!
+! The state lives on the GPU inside one engine handle; CalculateTendency and ForwardStep run
+! one fused HIP kernel per RK3 stage (hipGraph replay).  Host copies of the solution are
+! made only by UpdateHost.  Initial conditions, topography, bathymetry, Coriolis and gravity
+! fields are given as arrays (the reference's FEQParse equation strings are not used).
+MODULE SELF_Model2D
+  USE SELF_Constants
+  USE SELF_AMD_CAPI
+  USE SELF_Lagrange
+  USE SELF_Mesh
+  USE SELF_MappedData
+  IMPLICIT NONE
+
+  INTEGER, PARAMETER :: SELF_MODEL_LSW = 1, SELF_MODEL_SW = 2, SELF_MODEL_EULER = 3
+  INTEGER(C_INT), PARAMETER :: PARAM_G = 1, PARAM_FCORI = 2, PARAM_GAMMA = 3, PARAM_RGAS = 4
+  INTEGER(C_INT), PARAMETER :: FIELD_TOPOGRAPHY = 1, FIELD_BATHYMETRY = 2, FIELD_CORIOLIS = 3
+  INTEGER(C_INT), PARAMETER :: FIELD_POTENTIAL = 4, FIELD_DRAG = 5
+
+  TYPE, PUBLIC :: Model2D
+    INTEGER :: kind = 0
+    REAL(prec) :: t = 0.0_prec
+    REAL(prec) :: dt = 0.0_prec
+    TYPE(MappedScalar2D) :: solution
+    TYPE(MappedScalar2D) :: dSdt
+    TYPE(Mesh2D), POINTER :: mesh => NULL()
+    TYPE(SEMQuad), POINTER :: geometry => NULL()
+    TYPE(C_PTR) :: engine = C_NULL_PTR
+  CONTAINS
+    PROCEDURE, PUBLIC :: Init_Model2D
+    PROCEDURE, PUBLIC :: Free => Free_Model2D
+    PROCEDURE, PUBLIC :: SetSolution => SetSolution_Model2D
+    PROCEDURE, PUBLIC :: SetTimeIntegrator => SetTimeIntegrator_Model2D
+    PROCEDURE, PUBLIC :: CalculateTendency => CalculateTendency_Model2D
+    PROCEDURE, PUBLIC :: ForwardStep => ForwardStep_Model2D
+    PROCEDURE, PUBLIC :: UpdateHost => UpdateHost_Model2D
+    PROCEDURE, PRIVATE :: SetField => SetField_Model2D
+  END TYPE Model2D
+
+  TYPE, EXTENDS(Model2D), PUBLIC :: ShallowWater
+    REAL(prec) :: g = 1.0_prec
+    REAL(prec) :: fCori = 0.0_prec
+  CONTAINS
+    PROCEDURE, PUBLIC :: Init => Init_ShallowWater
+    PROCEDURE, PUBLIC :: SetTopography => SetTopography_ShallowWater
+  END TYPE ShallowWater
+
+  TYPE, EXTENDS(Model2D), PUBLIC :: LinearShallowWater
+    REAL(prec) :: g = 1.0_prec
+  CONTAINS
+    PROCEDURE, PUBLIC :: Init => Init_LinearShallowWater
+    PROCEDURE, PUBLIC :: SetBathymetry => SetBathymetry_LinearShallowWater
+    PROCEDURE, PUBLIC :: SetCoriolis => SetCoriolis_LinearShallowWater
+  END TYPE LinearShallowWater
+
+  TYPE, EXTENDS(Model2D), PUBLIC :: CompressibleIdealGas2D
+    REAL(prec) :: expansionFactor = 1.4_prec
+    REAL(prec) :: R = 1.0_prec
+  CONTAINS
+    PROCEDURE, PUBLIC :: Init => Init_CompressibleIdealGas2D
+    PROCEDURE, PUBLIC :: SetGravity => SetGravity_CompressibleIdealGas2D
+    PROCEDURE, PUBLIC :: SetDrag => SetDrag_CompressibleIdealGas2D
+  END TYPE CompressibleIdealGas2D
+
+CONTAINS
+
+  ! Init_Model2D (src/SELF_Model2D.f90:142-178)
+  SUBROUTINE Init_Model2D(this, kind, nvar, mesh, geometry)
+    CLASS(Model2D), INTENT(inout) :: this
+    INTEGER, INTENT(in) :: kind, nvar
+    TYPE(Mesh2D), INTENT(in), TARGET :: mesh
+    TYPE(SEMQuad), INTENT(in), TARGET :: geometry
+    this % kind = kind
+    this % mesh => mesh
+    this % geometry => geometry
+    this % t = 0.0_prec
+    CALL this % solution % Init(geometry % interp, nvar, mesh % nElem)
+    CALL this % dSdt % Init(geometry % interp, nvar, mesh % nElem)
+    CALL selfamd_check(selfamd_model2d_create(this % engine, INT(kind, C_INT), INT(geometry % interp % N, C_INT), &
+                       INT(mesh % nElem, C_INT), this % mesh % sideInfo, geometry % dsdx, geometry % J, &
+                       geometry % nHatb, geometry % nScaleb, C_NULL_PTR), 'Model2D%Init')
+  END SUBROUTINE Init_Model2D
+
+  SUBROUTINE Free_Model2D(this)
+    CLASS(Model2D), INTENT(inout) :: this
+    IF (C_ASSOCIATED(this % engine)) CALL selfamd_check(selfamd_model2d_destroy(this % engine), 'Model2D%Free')
+    this % engine = C_NULL_PTR
+    CALL this % solution % Free()
+    CALL this % dSdt % Free()
+  END SUBROUTINE Free_Model2D
+
+  ! SetSolution (src/SELF_Model2D.f90:313-332) from an array interior(0:N,0:N,nVar,nElem)
+  SUBROUTINE SetSolution_Model2D(this, interior)
+    CLASS(Model2D), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: interior(:,:,:,:)
+    this % solution % interior = interior
+    CALL this % solution % UpdateDevice()
+    CALL selfamd_check(selfamd_model2d_set_solution(this % engine, this % solution % interior_dev), &
+                       'Model2D%SetSolution')
+  END SUBROUTINE SetSolution_Model2D
+
+  ! SetTimeIntegrator (src/SELF_Model.f90:292-382): the engine implements Williamson's
+  ! low-storage RK3 (the integrator of the tendency path)
+  SUBROUTINE SetTimeIntegrator_Model2D(this, integrator)
+    CLASS(Model2D), INTENT(inout) :: this
+    CHARACTER(*), INTENT(in) :: integrator
+    IF (TRIM(integrator) /= 'rk3' .AND. TRIM(integrator) /= 'RK3') THEN
+      PRINT '(A,A)', 'self_amd: unsupported time integrator ', TRIM(integrator)
+      ERROR STOP 2
+    END IF
+  END SUBROUTINE SetTimeIntegrator_Model2D
+
+  ! CalculateTendency_Model2D (src/SELF_Model2D.f90:816-859): dSdt on the device
+  SUBROUTINE CalculateTendency_Model2D(this)
+    CLASS(Model2D), INTENT(inout) :: this
+    CALL selfamd_check(selfamd_model2d_tendency(this % engine, this % dSdt % interior_dev), &
+                       'Model2D%CalculateTendency')
+  END SUBROUTINE CalculateTendency_Model2D
+
+  ! ForwardStep_Model + LowStorageRK3_timeIntegrator (src/SELF_Model.f90:474-523, 704-732):
+  ! the reference's step loop dt = MIN(dtLim, tn - t) is replayed on the host, full steps are
+  ! batched into one engine call and a shorter final step gets its own call.
+  SUBROUTINE ForwardStep_Model2D(this, tn, dt)
+    CLASS(Model2D), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: tn
+    REAL(prec), OPTIONAL, INTENT(in) :: dt
+    REAL(prec) :: t, dtStep
+    INTEGER :: nFull
+    IF (PRESENT(dt)) this % dt = dt
+    t = this % t
+    nFull = 0
+    DO WHILE (t < tn)
+      dtStep = MIN(this % dt, tn - t)
+      IF (dtStep == this % dt) THEN
+        nFull = nFull + 1
+      ELSE
+        IF (nFull > 0) CALL selfamd_check(selfamd_model2d_forward_step(this % engine, INT(nFull, C_INT), this % dt), &
+                                          'Model2D%ForwardStep')
+        nFull = 0
+        CALL selfamd_check(selfamd_model2d_forward_step(this % engine, 1_C_INT, dtStep), 'Model2D%ForwardStep')
+      END IF
+      t = t + dtStep
+    END DO
+    IF (nFull > 0) CALL selfamd_check(selfamd_model2d_forward_step(this % engine, INT(nFull, C_INT), this % dt), &
+                                      'Model2D%ForwardStep')
+    CALL selfamd_check(selfamd_model2d_synchronize(this % engine), 'Model2D%ForwardStep')
+    this % t = t
+    CALL selfamd_check(selfamd_model2d_set_time(this % engine, t), 'Model2D%ForwardStep')
+  END SUBROUTINE ForwardStep_Model2D
+
+  ! Solution (and the last tendency) back to the host mirrors
+  SUBROUTINE UpdateHost_Model2D(this)
+    CLASS(Model2D), INTENT(inout), TARGET :: this
+    CALL selfamd_check(selfamd_model2d_get_solution(this % engine, this % solution % interior_dev), &
+                       'Model2D%UpdateHost')
+    CALL hip_check(hipMemcpy(C_LOC(this % solution % interior), this % solution % interior_dev, &
+                   INT(SIZE(this % solution % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
+    CALL hip_check(hipMemcpy(C_LOC(this % dSdt % interior), this % dSdt % interior_dev, &
+                   INT(SIZE(this % dSdt % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
+  END SUBROUTINE UpdateHost_Model2D
+
+  SUBROUTINE SetField_Model2D(this, which, values)
+    CLASS(Model2D), INTENT(inout) :: this
+    INTEGER(C_INT), INTENT(in) :: which
+    REAL(prec), INTENT(in), TARGET :: values(:,:,:)
+    TYPE(C_PTR) :: d
+    INTEGER(C_SIZE_T) :: nb
+    nb = INT(SIZE(values), C_SIZE_T)*8_C_SIZE_T
+    d = device_alloc(nb)
+    CALL hip_check(hipMemcpy(d, C_LOC(values), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+    CALL selfamd_check(selfamd_model2d_set_field(this % engine, which, d), 'Model2D%SetField')
+    CALL device_free(d)
+  END SUBROUTINE SetField_Model2D
+
+  ! ---- ShallowWater (src/models/SELF_ShallowWater.f90:96-147) ----
+  SUBROUTINE Init_ShallowWater(this, nvar, mesh, geometry)
+    CLASS(ShallowWater), INTENT(inout) :: this
+    INTEGER, INTENT(in) :: nvar
+    TYPE(Mesh2D), INTENT(in), TARGET :: mesh
+    TYPE(SEMQuad), INTENT(in), TARGET :: geometry
+    CALL this % Init_Model2D(SELF_MODEL_SW, 3, mesh, geometry)
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_G, this % g), 'ShallowWater%Init')
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_FCORI, this % fCori), 'ShallowWater%Init')
+  END SUBROUTINE Init_ShallowWater
+
+  ! SetTopography (src/models/SELF_ShallowWater.f90:241-282), topography(0:N,0:N,nElem)
+  SUBROUTINE SetTopography_ShallowWater(this, topography)
+    CLASS(ShallowWater), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: topography(:,:,:)
+    CALL this % SetField(FIELD_TOPOGRAPHY, topography)
+  END SUBROUTINE SetTopography_ShallowWater
+
+  ! ---- LinearShallowWater (src/models/SELF_LinearShallowWater.f90:101-151) ----
+  SUBROUTINE Init_LinearShallowWater(this, nvar, mesh, geometry)
+    CLASS(LinearShallowWater), INTENT(inout) :: this
+    INTEGER, INTENT(in) :: nvar
+    TYPE(Mesh2D), INTENT(in), TARGET :: mesh
+    TYPE(SEMQuad), INTENT(in), TARGET :: geometry
+    REAL(prec), ALLOCATABLE :: zero(:,:,:)
+    CALL this % Init_Model2D(SELF_MODEL_LSW, 3, mesh, geometry)
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_G, this % g), 'LinearShallowWater%Init')
+    ALLOCATE (zero(0:geometry % interp % N, 0:geometry % interp % N, mesh % nElem))
+    zero = 0.0_prec
+    CALL this % SetCoriolis(zero)  ! fCori starts at zero (hfReal allocation zero-fills)
+  END SUBROUTINE Init_LinearShallowWater
+
+  SUBROUTINE SetBathymetry_LinearShallowWater(this, H)
+    CLASS(LinearShallowWater), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: H(:,:,:)
+    CALL this % SetField(FIELD_BATHYMETRY, H)
+  END SUBROUTINE SetBathymetry_LinearShallowWater
+
+  SUBROUTINE SetCoriolis_LinearShallowWater(this, f)
+    CLASS(LinearShallowWater), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: f(:,:,:)
+    CALL this % SetField(FIELD_CORIOLIS, f)
+  END SUBROUTINE SetCoriolis_LinearShallowWater
+
+  ! ---- CompressibleIdealGas2D (src/models/SELF_CompressibleIdealGas2D.f90:261-358) ----
+  SUBROUTINE Init_CompressibleIdealGas2D(this, nvar, mesh, geometry)
+    CLASS(CompressibleIdealGas2D), INTENT(inout) :: this
+    INTEGER, INTENT(in) :: nvar
+    TYPE(Mesh2D), INTENT(in), TARGET :: mesh
+    TYPE(SEMQuad), INTENT(in), TARGET :: geometry
+    CALL this % Init_Model2D(SELF_MODEL_EULER, 4, mesh, geometry)
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_GAMMA, this % expansionFactor), &
+                       'CompressibleIdealGas2D%Init')
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_RGAS, this % R), 'CompressibleIdealGas2D%Init')
+  END SUBROUTINE Init_CompressibleIdealGas2D
+
+  SUBROUTINE SetGravity_CompressibleIdealGas2D(this, potential)
+    CLASS(CompressibleIdealGas2D), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: potential(:,:,:)
+    CALL this % SetField(FIELD_POTENTIAL, potential)
+  END SUBROUTINE SetGravity_CompressibleIdealGas2D
+
+  SUBROUTINE SetDrag_CompressibleIdealGas2D(this, Cd)
+    CLASS(CompressibleIdealGas2D), INTENT(inout) :: this
+    REAL(prec), INTENT(in) :: Cd(:,:,:)
+    CALL this % SetField(FIELD_DRAG, Cd)
+  END SUBROUTINE SetDrag_CompressibleIdealGas2D
+
+END MODULE SELF_Model2D

@@ -1,0 +1,236 @@
+! self_amd_capi.f90 -- iso_c_binding interfaces to libself_amd.so (include/self_amd.h) and the
+! few HIP runtime calls the Fortran host layer needs (device allocation and copies).
+!
+! Device pointers are passed BY VALUE (type(c_ptr), value); every entry point returns an
+! integer status (0 = success).  selfamd_check stops with the library's message otherwise.
+MODULE SELF_AMD_CAPI
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+
+  INTEGER(C_INT), PARAMETER :: hipMemcpyHostToDevice = 1
+  INTEGER(C_INT), PARAMETER :: hipMemcpyDeviceToHost = 2
+  INTEGER(C_INT), PARAMETER :: hipMemcpyDeviceToDevice = 3
+
+  INTERFACE
+    ! ---- HIP runtime (libamdhip64) ----
+    INTEGER(C_INT) FUNCTION hipMalloc(ptr, bytes) BIND(C, name="hipMalloc")
+      IMPORT :: C_INT, C_PTR, C_SIZE_T
+      TYPE(C_PTR) :: ptr
+      INTEGER(C_SIZE_T), VALUE :: bytes
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION hipFree(ptr) BIND(C, name="hipFree")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ptr
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION hipMemcpy(dst, src, bytes, kind) BIND(C, name="hipMemcpy")
+      IMPORT :: C_INT, C_PTR, C_SIZE_T
+      TYPE(C_PTR), VALUE :: dst, src
+      INTEGER(C_SIZE_T), VALUE :: bytes
+      INTEGER(C_INT), VALUE :: kind
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION hipMemset(ptr, val, bytes) BIND(C, name="hipMemset")
+      IMPORT :: C_INT, C_PTR, C_SIZE_T
+      TYPE(C_PTR), VALUE :: ptr
+      INTEGER(C_INT), VALUE :: val
+      INTEGER(C_SIZE_T), VALUE :: bytes
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION hipDeviceSynchronize() BIND(C, name="hipDeviceSynchronize")
+      IMPORT :: C_INT
+    END FUNCTION
+
+    ! ---- libself_amd: errors ----
+    TYPE(C_PTR) FUNCTION selfamd_last_error() BIND(C, name="selfamd_last_error")
+      IMPORT :: C_PTR
+    END FUNCTION
+
+    ! ---- host setup ----
+    INTEGER(C_INT) FUNCTION selfamd_lagrange_init(N, ctrl, M, tgt, cp, tp, bw, qw, iM, dM, dgM, bM) &
+        BIND(C, name="selfamd_lagrange_init")
+      IMPORT :: C_INT, C_DOUBLE
+      INTEGER(C_INT), VALUE :: N, ctrl, M, tgt
+      REAL(C_DOUBLE) :: cp(*), tp(*), bw(*), qw(*), iM(*), dM(*), dgM(*), bM(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_mesh2d_structured(nx, ny, x0, x1, y0, y1, px, py, bcid, sideInfo, &
+        nodeCoords, gnid) BIND(C, name="selfamd_mesh2d_structured")
+      IMPORT :: C_INT, C_DOUBLE, C_INT32_T
+      INTEGER(C_INT), VALUE :: nx, ny, px, py, bcid
+      REAL(C_DOUBLE), VALUE :: x0, x1, y0, y1
+      INTEGER(C_INT32_T) :: sideInfo(*), gnid(*)
+      REAL(C_DOUBLE) :: nodeCoords(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_mesh2d_recalculate_flip(nElem, nGeo, sideInfo, gnid) &
+        BIND(C, name="selfamd_mesh2d_recalculate_flip")
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT), VALUE :: nElem, nGeo
+      INTEGER(C_INT32_T) :: sideInfo(*), gnid(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_mesh2d_reset_boundary_condition_type(nElem, sideInfo, bcid) &
+        BIND(C, name="selfamd_mesh2d_reset_boundary_condition_type")
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT), VALUE :: nElem, bcid
+      INTEGER(C_INT32_T) :: sideInfo(*)
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_semquad_generate(N, ctrl, nGeo, nEl, nodeCoords, x, xb, dxds, dxdsb, &
+        dsdx, dsdxb, nHatb, nScaleb, J, Jb, stream) BIND(C, name="selfamd_semquad_generate")
+      IMPORT :: C_INT, C_PTR
+      INTEGER(C_INT), VALUE :: N, ctrl, nGeo, nEl
+      TYPE(C_PTR), VALUE :: nodeCoords, x, xb, dxds, dxdsb, dsdx, dsdxb, nHatb, nScaleb, J, Jb, stream
+    END FUNCTION
+
+    ! ---- operators (device) ----
+    INTEGER(C_INT) FUNCTION selfamd_scalar_boundary_interp_2d(bM, f, fb, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_scalar_boundary_interp_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: bM, f, fb, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_vector_boundary_interp_2d(bM, f, fb, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_vector_boundary_interp_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: bM, f, fb, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_scalar_grid_interp_2d(iM, f, fNew, N, M, nVar, nEl, stream) &
+        BIND(C, name="selfamd_scalar_grid_interp_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: iM, f, fNew, stream
+      INTEGER(C_INT), VALUE :: N, M, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_vector_dg_divergence_2d(dgM, bM, qW, f, bF, dF, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_vector_dg_divergence_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: dgM, bM, qW, f, bF, dF, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_side_exchange_scalar_2d(b, ext, sideInfo, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_side_exchange_scalar_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: b, ext, sideInfo, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_side_exchange_vector_2d(b, ext, sideInfo, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_side_exchange_vector_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: b, ext, sideInfo, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_gradient_sf_2d(dM, dsdx, J, f, gradF, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_gradient_sf_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: dM, dsdx, J, f, gradF, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_jacobian_weight_2d(J, f, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_jacobian_weight_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: J, f, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_contravariant_projection_2d(dsdx, f, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_contravariant_projection_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: dsdx, f, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+
+    ! ---- Model2D engine ----
+    INTEGER(C_INT) FUNCTION selfamd_model2d_create(model, kind, N, nEl, sideInfo, dsdx, J, nHatb, nScaleb, &
+        stream) BIND(C, name="selfamd_model2d_create")
+      IMPORT :: C_INT, C_PTR, C_INT32_T
+      TYPE(C_PTR) :: model
+      INTEGER(C_INT), VALUE :: kind, N, nEl
+      INTEGER(C_INT32_T) :: sideInfo(*)
+      TYPE(C_PTR), VALUE :: dsdx, J, nHatb, nScaleb, stream
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_destroy(model) BIND(C, name="selfamd_model2d_destroy")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_set_param(model, which, value) BIND(C, name="selfamd_model2d_set_param")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: model
+      INTEGER(C_INT), VALUE :: which
+      REAL(C_DOUBLE), VALUE :: value
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_set_field(model, which, field) BIND(C, name="selfamd_model2d_set_field")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model, field
+      INTEGER(C_INT), VALUE :: which
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_set_solution(model, interior) &
+        BIND(C, name="selfamd_model2d_set_solution")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model, interior
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_get_solution(model, interior) &
+        BIND(C, name="selfamd_model2d_get_solution")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model, interior
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_tendency(model, dSdt) BIND(C, name="selfamd_model2d_tendency")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model, dSdt
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_forward_step(model, nSteps, dt) &
+        BIND(C, name="selfamd_model2d_forward_step")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: model
+      INTEGER(C_INT), VALUE :: nSteps
+      REAL(C_DOUBLE), VALUE :: dt
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_synchronize(model) BIND(C, name="selfamd_model2d_synchronize")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_set_time(model, t) BIND(C, name="selfamd_model2d_set_time")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: model
+      REAL(C_DOUBLE), VALUE :: t
+    END FUNCTION
+  END INTERFACE
+
+CONTAINS
+
+  SUBROUTINE selfamd_check(rc, what)
+    !! Stop with the library's error message when a call failed
+    INTEGER(C_INT), INTENT(in) :: rc
+    CHARACTER(*), INTENT(in) :: what
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    CHARACTER(LEN=512) :: text
+    INTEGER :: k
+    IF (rc == 0) RETURN
+    CALL C_F_POINTER(selfamd_last_error(), msg, [512])
+    text = ''
+    DO k = 1, 512
+      IF (msg(k) == C_NULL_CHAR) EXIT
+      text(k:k) = msg(k)
+    END DO
+    PRINT '(A,A,A,I0,A,A)', 'self_amd: ', what, ' failed (status ', rc, '): ', TRIM(text)
+    ERROR STOP 3
+  END SUBROUTINE selfamd_check
+
+  SUBROUTINE hip_check(rc, what)
+    INTEGER(C_INT), INTENT(in) :: rc
+    CHARACTER(*), INTENT(in) :: what
+    IF (rc /= 0) THEN
+      PRINT '(A,A,A,I0)', 'self_amd: HIP call ', what, ' failed with hipError ', rc
+      ERROR STOP 3
+    END IF
+  END SUBROUTINE hip_check
+
+  FUNCTION device_alloc(nbytes) RESULT(p)
+    !! hipMalloc + zero fill
+    INTEGER(C_SIZE_T), INTENT(in) :: nbytes
+    TYPE(C_PTR) :: p
+    p = C_NULL_PTR
+    CALL hip_check(hipMalloc(p, MAX(nbytes, 8_C_SIZE_T)), 'hipMalloc')
+    CALL hip_check(hipMemset(p, 0_C_INT, MAX(nbytes, 8_C_SIZE_T)), 'hipMemset')
+  END FUNCTION device_alloc
+
+  SUBROUTINE device_free(p)
+    TYPE(C_PTR), INTENT(inout) :: p
+    IF (C_ASSOCIATED(p)) CALL hip_check(hipFree(p), 'hipFree')
+    p = C_NULL_PTR
+  END SUBROUTINE device_free
+
+END MODULE SELF_AMD_CAPI

@@ -29,12 +29,12 @@ def _id(c):
 def test_tendency_parity(kind, N, n, periodic, bcid):
     c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=3)
     m, _ = c.gpu_model()
-    d = m.CalculateTendency().cpu().numpy()
+    d = m.CalculateTendency()
     ref = c.om.tendency()
     err = rel_err(d, ref)
     assert err.max() <= TOL, err
     # a second call sees the first call's velocity traces (SW lag) on both sides
-    d2 = m.CalculateTendency().cpu().numpy()
+    d2 = m.CalculateTendency()
     assert rel_err(d2, c.om.tendency()).max() <= TOL
 
 
@@ -46,7 +46,7 @@ def test_rk3_trajectory_parity(kind, N, n, periodic, bcid):
     for chunk in (3, 7):  # two ForwardStep calls: the engine's state persists between them
         m.ForwardStep(chunk, dt)
         c.om.rk3_steps(chunk, dt)
-        err = rel_err(m.solution.cpu().numpy(), c.om.solution)
+        err = rel_err(m.solution, c.om.solution)
         assert err.max() <= TOL, err
     assert abs(m.t - c.om.get_scalar("t")) <= 1e-12 * abs(m.t)
 
@@ -59,7 +59,7 @@ def test_tendency_then_steps_keeps_lag_state():
     dt = c.dt()
     m.ForwardStep(4, dt)
     c.om.rk3_steps(4, dt)
-    assert rel_err(m.solution.cpu().numpy(), c.om.solution).max() <= TOL
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
 def test_hopr_block_mesh_parity():
@@ -74,10 +74,10 @@ def test_hopr_block_mesh_parity():
     mesh.ResetBoundaryConditionType(self_amd.BC_NONORMALFLOW)
     c = Case("lsw", 7, 0, 0, hopr=mesh, seed=2)
     m, _ = c.gpu_model()
-    assert rel_err(m.CalculateTendency().cpu().numpy(), c.om.tendency()).max() <= TOL
+    assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
     m.ForwardStep(5, c.dt())
     c.om.rk3_steps(5, c.dt())
-    assert rel_err(m.solution.cpu().numpy(), c.om.solution).max() <= TOL
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
 def test_full_size_sw_256():
@@ -93,13 +93,13 @@ def test_full_size_sw_256():
     finally:
         oracle.set_threads(1)
     m.ForwardStep(2, dt)
-    assert rel_err(m.solution.cpu().numpy(), c.om.solution).max() <= TOL
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
     w = c.weights
-    J = geom.J.cpu().numpy()
+    J = geom.J.to_host()
     def mass(s):
         return float(np.sum(s[:, 2] * J * w[None, :, None] * w[None, None, :]))
     m0 = mass(c.sol)
     m.ForwardStep(20, dt)
-    s = m.solution.cpu().numpy()
+    s = m.solution
     assert np.isfinite(s).all()
     assert abs(mass(s) - m0) <= 1e-13 * m0

@@ -15,34 +15,31 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _t():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
-    return torch
+def dev(a, dtype=np.float64):
+    assert self_amd.device_count() >= 1, "GPU tests need a ROCm device"
+    return self_amd.DeviceArray.from_host(np.ascontiguousarray(a, dtype=dtype))
 
 
-def dev(a, dtype=None):
-    torch = _t()
-    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dtype or torch.float64)
-
-
-def zeros(shape, like=None):
-    torch = _t()
-    return torch.zeros(shape, dtype=torch.float64, device="cuda")
+def zeros(shape):
+    return self_amd.DeviceArray(shape, zero=True)
 
 
 def p(t):
-    return ctypes.c_void_p(t.data_ptr())
+    """Argument marker: the DeviceArray itself travels to run(), which keeps it referenced
+    (and so allocated) until the call has completed."""
+    assert isinstance(t, self_amd.DeviceArray)
+    return t
 
 
 def run(name, *args):
-    torch = _t()
-    _lib.call(name, *args, None)
-    torch.cuda.synchronize()
+    held = args  # device arrays stay alive for the whole call
+    _lib.call(name, *[a.ptr if isinstance(a, self_amd.DeviceArray) else a for a in held], None)
+    self_amd.synchronize()
+    del held
 
 
 def host(t):
-    return t.cpu().numpy()
+    return t.to_host()
 
 
 @pytest.fixture(scope="module")
@@ -113,7 +110,7 @@ def test_side_exchange_bit_exact(periodic, rng):
     nEl = mesh.nElem
     b = rng.normal(size=(nEl, 4, nVar, N + 1))
     ext = zeros(b.shape)
-    si = dev(mesh.sideInfo, dtype=_t().int32)
+    si = dev(mesh.sideInfo, dtype=np.int32)
     run("selfamd_side_exchange_scalar_2d", p(dev(b)), p(ext), p(si), N, nVar, nEl)
     np.testing.assert_array_equal(host(ext), oracle.side_exchange_scalar(N, mesh.sideInfo, b))
     bv = rng.normal(size=(nEl, 4, 1, N + 1, 2))
@@ -139,7 +136,7 @@ def test_geometry_generation(source):
         g = self_amd.SEMQuad(self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS), mesh)
         og = oracle.Geometry(N, mesh.nodeCoords, mesh.nGeo)
         for k in ("x", "xb", "dxds", "dxdsb", "dsdx", "dsdxb", "nHatb", "nScaleb", "J", "Jb"):
-            a, b = host(getattr(g, k)), getattr(og, k)
+            a, b = getattr(g, k).to_host(), getattr(og, k)
             err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
             assert err <= TOL, (k, N, err)
         assert abs(g.CovariantArcMin() - og.covariant_arc_min()) <= 1e-14 * og.covariant_arc_min()
@@ -194,7 +191,7 @@ def test_sw_physics_operators(bcid):
     # feed the oracle's velB source (solB of the set-solution call is the same data here)
     run("selfamd_sw2d_velocity", p(sol), p(solB), p(vel), p(velB), N, nEl)
     assert rel_err(host(vel)[None], F("vel", (nEl, P, P, 2))[None], 0).max() <= TOL
-    si = dev(c.mesh.sideInfo, dtype=_t().int32)
+    si = dev(c.mesh.sideInfo, dtype=np.int32)
     velE = zeros((nEl, 4, P, 2))
     run("selfamd_side_exchange_vector_2d", p(velB), p(velE), p(si), N, 1, nEl)
     np.testing.assert_array_equal(host(velE), F("velE", (nEl, 4, P, 2)))
@@ -221,7 +218,7 @@ def test_lsw_physics_operators():
     F = lambda n, s: om.field(n, s).copy()
     ish, bsh = om.interior_shape, om.boundary_shape
     sol, solB = dev(F("sol", ish)), dev(F("solB", bsh))
-    si = dev(c.mesh.sideInfo, dtype=_t().int32)
+    si = dev(c.mesh.sideInfo, dtype=np.int32)
     ext = dev(F("solE", bsh) * (c.mesh.sideInfo[..., 2][:, :, None, None] != 0))
     run("selfamd_lsw2d_set_boundary_condition", p(solB), p(ext), p(dev(c.geom.nHatb)), p(si), N, nEl)
     assert rel_err(host(ext), F("solE", bsh), 2).max() <= TOL
@@ -254,7 +251,7 @@ def test_euler_physics_operators(bcid):
         out = zeros(ish)
         run(name, p(sol), p(out), *args, N, nEl)
         assert rel_err(host(out), F(fld, ish)).max() <= TOL, name
-    si = dev(c.mesh.sideInfo, dtype=_t().int32)
+    si = dev(c.mesh.sideInfo, dtype=np.int32)
     mask = (c.mesh.sideInfo[..., 2] != 0)[:, :, None, None]
     solE, primE, diagE = dev(F("solE", bsh) * mask), dev(F("primE", bsh) * mask), dev(F("diagE", bsh) * mask)
     run("selfamd_euler2d_set_boundary_condition", p(dev(F("solB", bsh))), p(solE), p(dev(F("presSolB", bsh))),
