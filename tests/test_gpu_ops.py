@@ -139,7 +139,7 @@ def test_geometry_generation(source):
             a, b = getattr(g, k).to_host(), getattr(og, k)
             err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
             assert err <= TOL, (k, N, err)
-        assert abs(g.CovariantArcMin() - og.covariant_arc_min()) <= 1e-14 * og.covariant_arc_min()
+        assert abs(g.CovariantArcMin() - og.covariant_arc_min()) <= TOL * og.covariant_arc_min()
 
 
 def test_mapped_operators(rng):
