@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--N", type=int, default=7)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="RK3 steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=100, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 

@@ -252,6 +252,8 @@ typedef struct selfamd_model2d selfamd_model2d;
 #define SELFAMD_PARAM_FCORI 2  /* SW constant Coriolis parameter */
 #define SELFAMD_PARAM_GAMMA 3  /* Euler expansion factor Cp/Cv */
 #define SELFAMD_PARAM_RGAS 4   /* Euler gas constant */
+#define SELFAMD_PARAM_NODAL_METRICS 5  /* 1: per-node metric terms (as SEMQuad stores them);
+                                          0: one affine record per element (affine meshes) */
 /* static per-node fields (selfamd_model2d_set_field), interior(0:N,0:N,1,nEl) device arrays */
 #define SELFAMD_FIELD_TOPOGRAPHY 1  /* SW this%H (SetTopography) */
 #define SELFAMD_FIELD_BATHYMETRY 2  /* LSW this%H (SetBathymetry) */
@@ -261,8 +263,9 @@ typedef struct selfamd_model2d selfamd_model2d;
 
 /* Init_Model2D + the model's Init (e.g. src/models/SELF_ShallowWater.f90:96-147).  sideInfo is
  * host memory; geometry arrays are SEMQuad device arrays (dsdx interior, J interior, nHat and
- * nScale boundary) from selfamd_semquad_generate.  Fails with SELFAMD_ENOTAFFINE when the
- * metric terms are not constant per element (curved meshes run through the operator API). */
+ * nScale boundary) from selfamd_semquad_generate, copied into the engine.  Affine meshes
+ * (metric terms constant per element) default to one metric record per element; other
+ * meshes use the per-node metrics (see SELFAMD_PARAM_NODAL_METRICS). */
 int selfamd_model2d_create(selfamd_model2d **model, int kind, int N, int nEl, const int32_t *sideInfo,
                            const double *dsdx_dev, const double *J_dev, const double *nHatb_dev,
                            const double *nScaleb_dev, void *stream);

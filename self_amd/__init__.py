@@ -21,7 +21,7 @@ from ._lib import SelfAmdError, call, load
 GAUSS, GAUSS_LOBATTO, UNIFORM = 1, 2, 5
 BC_PRESCRIBED, BC_RADIATION, BC_NONORMALFLOW = 100, 101, 102
 MODEL_LSW, MODEL_SW, MODEL_EULER = 1, 2, 3
-PARAM_G, PARAM_FCORI, PARAM_GAMMA, PARAM_RGAS = 1, 2, 3, 4
+PARAM_G, PARAM_FCORI, PARAM_GAMMA, PARAM_RGAS, PARAM_NODAL_METRICS = 1, 2, 3, 4, 5
 FIELD_TOPOGRAPHY, FIELD_BATHYMETRY, FIELD_CORIOLIS, FIELD_POTENTIAL, FIELD_DRAG = 1, 2, 3, 4, 5
 
 __all__ = ["DeviceArray", "device_count", "device_name", "synchronize", "Lagrange", "Mesh2D", "SEMQuad",
@@ -267,6 +267,11 @@ class Model2D:
     def _field(self, which, values):
         f = _dev(values, (self.nElem, self.N + 1, self.N + 1))
         call("selfamd_model2d_set_field", self._h, which, _ptr(f))
+
+    def SetNodalMetrics(self, on=True):
+        """Use per-node metric terms (the reference's representation; required on curved
+        meshes) instead of one affine record per element."""
+        self._param(PARAM_NODAL_METRICS, 1.0 if on else 0.0)
 
     def SetSolution(self, interior):
         """SetSolution (src/SELF_Model2D.f90:313-332) from interior[e, v, j, i] (numpy or DeviceArray)."""

@@ -52,7 +52,8 @@ struct FusedArgs {
   const double *aux;        // SW topography / LSW bathymetry / Euler potential, or null
   const double *stat2;      // LSW Coriolis / Euler drag, or null
   const double *presTrace;  // Euler prescribed traces (0:N,9,4,nEl), or null
-  const double *geom;       // per-element records
+  const double *geom;       // per-element records (affine mode)
+  const double *ndsdx, *nJ, *nnHat, *nnScale;  // per-node metrics (nodal mode), SEMQuad shapes
   const int32_t *nbr;       // per (side, element) neighbour code
   const double *mats;       // dgM(P*P) D(P*P) bM(2P) qW(P)
   int nEl;
@@ -230,7 +231,10 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
 // The fused kernel
 // ------------------------------------------------------------------------------
 // MODE 0: RK stage (update S, G, write next traces); MODE 1: tendency only (write dSdt).
-template <class Phys, int N, int MODE>
+// NODAL: metric terms per node and per face node exactly as SEMQuad stores them (curved
+// elements, and the reference's own per-node metric values); otherwise one affine record
+// per element.
+template <class Phys, int N, int MODE, bool NODAL>
 __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   constexpr int NP = N + 1, NN = NP * NP, EPB = kThreads / NN, NF = 4 * NP;
   constexpr int NV = Phys::NV, NT = Phys::NT;
@@ -239,6 +243,9 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   __shared__ double sF1[NV][EPB][NN];
   __shared__ double sF2[NV][EPB][NN];
   __shared__ double sAux[USE_AUX ? EPB : 1][USE_AUX ? NN : 1];
+  // nodal mode: static field times the node's contravariant metric, for GradientSF
+  constexpr bool GRADN = NODAL && Phys::kAuxGrad;
+  __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NN : 1];
   __shared__ double sBF[NV][EPB][NF];
   __shared__ double sM[2 * NN + 3 * NP];  // dgM, D, bM(2P), qW
 
@@ -256,9 +263,15 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   double statVal = a.p.stat2Const;
   double dsdx[4] = {0, 0, 0, 0}, jac = 1.0;
   if (nodeActive) {
-    const double *gr = a.geom + (size_t)e * kGeomStride;
-    dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
-    jac = gr[4];
+    if constexpr (NODAL) {
+      const double *gn = a.ndsdx + 4 * ((size_t)e * NN + node);
+      dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
+      jac = a.nJ[(size_t)e * NN + node];
+    } else {
+      const double *gr = a.geom + (size_t)e * kGeomStride;
+      dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
+      jac = gr[4];
+    }
 #pragma unroll
     for (int v = 0; v < NV; v++) s[v] = a.S[sBase + (size_t)v * NN + node];
     if (MODE == 0 && a.rkA != 0.0) {
@@ -282,6 +295,10 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
 #pragma unroll
     for (int k = 0; k < NT; k++) sQ[k][el][node] = q[k];
     if (USE_AUX) sAux[el][node] = auxVal;
+    if constexpr (GRADN) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) sGb[k][el][node] = auxVal * dsdx[k];
+    }
   }
   __syncthreads();
 
@@ -305,8 +322,14 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
       for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel][nd];
       if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel][nd];
     }
-    const double *gr = a.geom + (size_t)fe * kGeomStride;
-    const double nx = gr[8 + side], ny = gr[12 + side], nmag = gr[16 + side];
+    double nx, ny, nmag;
+    if constexpr (NODAL) {
+      const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
+      nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
+    } else {
+      const double *gr = a.geom + (size_t)fe * kGeomStride;
+      nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
+    }
     const int code = a.nbr[4 * (size_t)fe + side];
     bool interior = false;
     if (code >= 0) {
@@ -352,9 +375,14 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
     if (Phys::kAuxGrad && a.aux) {
       // GradientSF of the static field (src/SELF_MappedData.f90:1080-1149)
       for (int ii = 0; ii < NP; ii++) {
-        const double fa = sAux[el][ii + NP * j], fb = sAux[el][i + NP * ii];
-        gx = gx + Dm[ii + NP * i] * (fa * dsdx[0]) + Dm[ii + NP * j] * (fb * dsdx[2]);
-        gy = gy + Dm[ii + NP * i] * (fa * dsdx[1]) + Dm[ii + NP * j] * (fb * dsdx[3]);
+        if constexpr (GRADN) {
+          gx = gx + Dm[ii + NP * i] * sGb[0][el][ii + NP * j] + Dm[ii + NP * j] * sGb[2][el][i + NP * ii];
+          gy = gy + Dm[ii + NP * i] * sGb[1][el][ii + NP * j] + Dm[ii + NP * j] * sGb[3][el][i + NP * ii];
+        } else {
+          const double fa = sAux[el][ii + NP * j], fb = sAux[el][i + NP * ii];
+          gx = gx + Dm[ii + NP * i] * (fa * dsdx[0]) + Dm[ii + NP * j] * (fb * dsdx[2]);
+          gy = gy + Dm[ii + NP * i] * (fa * dsdx[1]) + Dm[ii + NP * j] * (fb * dsdx[3]);
+        }
       }
       gx = gx / jac;
       gy = gy / jac;
@@ -451,14 +479,20 @@ __global__ void __launch_bounds__(kThreads) k_traces(const double *S, double *tr
 // Initial SW lagged normal velocities from the current traces: at the first stage after
 // SetSolution the reference's velocity traces come from SetSolution's BoundaryInterp.
 template <int N>
-__global__ void k_sw_un_init(const double *trace, double *un, const double *geom, const int32_t *nbr, int nEl) {
+__global__ void k_sw_un_init(const double *trace, double *un, const double *geom, const double *nnHat,
+                             const int32_t *nbr, int nEl) {
   constexpr int NP = N + 1;
   const size_t total = (size_t)NP * 4 * nEl;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
     const int fi = t % NP, side = (t / NP) % 4;
     const size_t e = t / (4 * NP);
-    const double *gr = geom + e * kGeomStride;
-    const double nx = gr[8 + side], ny = gr[12 + side];
+    double nx, ny;
+    if (nnHat) {
+      nx = nnHat[2 * t]; ny = nnHat[2 * t + 1];
+    } else {
+      const double *gr = geom + e * kGeomStride;
+      nx = gr[8 + side]; ny = gr[12 + side];
+    }
     const double *tl = trace + fi + NP * 3 * (side + 4 * e);
     double *u = un + fi + NP * 2 * (side + 4 * e);
     u[0] = (tl[0] / tl[2 * NP]) * nx + (tl[NP] / tl[2 * NP]) * ny;
@@ -476,14 +510,19 @@ __global__ void k_sw_un_init(const double *trace, double *un, const double *geom
 
 template <class Phys>
 struct Launcher {
-  static hipError_t stage(int N, int mode, const FusedArgs &a, hipStream_t st) {
+  static hipError_t stage(int N, int mode, bool nodal, const FusedArgs &a, hipStream_t st) {
     switch (N) {
 #define CASE(NN_)                                                                                   \
   case NN_: {                                                                                       \
     constexpr int EPB = kThreads / ((NN_ + 1) * (NN_ + 1));                                         \
     dim3 grid((a.nEl + EPB - 1) / EPB);                                                             \
-    if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0>), grid, kThreads, 0, st, a);           \
-    else hipLaunchKernelGGL((k_fused<Phys, NN_, 1>), grid, kThreads, 0, st, a);                     \
+    if (nodal) {                                                                                    \
+      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, true>), grid, kThreads, 0, st, a);   \
+      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, true>), grid, kThreads, 0, st, a);             \
+    } else {                                                                                        \
+      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, false>), grid, kThreads, 0, st, a);  \
+      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, false>), grid, kThreads, 0, st, a);            \
+    }                                                                                               \
     return hipGetLastError();                                                                       \
   }
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
@@ -507,13 +546,13 @@ struct Launcher {
   }
 };
 
-hipError_t launch_un_init(int N, const double *tr, double *un, const double *geom, const int32_t *nbr, int nEl,
-                          hipStream_t st) {
+hipError_t launch_un_init(int N, const double *tr, double *un, const double *geom, const double *nnHat,
+                          const int32_t *nbr, int nEl, hipStream_t st) {
   const size_t n = (size_t)(N + 1) * 4 * nEl;
   const unsigned blocks = (unsigned)((n + 255) / 256);
   switch (N) {
 #define CASE(NN_) \
-  case NN_: hipLaunchKernelGGL((k_sw_un_init<NN_>), dim3(blocks), 256, 0, st, tr, un, geom, nbr, nEl); return hipGetLastError();
+  case NN_: hipLaunchKernelGGL((k_sw_un_init<NN_>), dim3(blocks), 256, 0, st, tr, un, geom, nnHat, nbr, nEl); return hipGetLastError();
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
     default: return hipErrorInvalidValue;
@@ -531,6 +570,9 @@ struct selfamd_model2d {
   bool ownStream = false;
   double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un = nullptr;
   double *geom = nullptr, *mats = nullptr, *aux = nullptr, *stat2 = nullptr, *presTrace = nullptr;
+  // per-node metrics (copied from SEMQuad at creation); used when `nodal` is set
+  double *ndsdx = nullptr, *nJ = nullptr, *nnHat = nullptr, *nnScale = nullptr;
+  bool affine = true, nodal = false;
   int32_t *nbr = nullptr;
   int cur = 0;  // trace buffer holding the traces of the current S
   Params p{};
@@ -546,13 +588,14 @@ struct selfamd_model2d {
     a.traceOut = trace[traceIn ^ 1];
     a.un = un; a.aux = aux; a.stat2 = stat2; a.presTrace = presTrace;
     a.geom = geom; a.nbr = nbr; a.mats = mats; a.nEl = nEl;
+    a.ndsdx = ndsdx; a.nJ = nJ; a.nnHat = nnHat; a.nnScale = nnScale;
     a.rkA = kRk3A[stage];
     a.rkGdt = kRk3G[stage] * dt;
     a.p = p;
     switch (kind) {
-      case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(N, mode, a, stream);
-      case SELFAMD_MODEL_LSW: return Launcher<PhysLSW>::stage(N, mode, a, stream);
-      default: return Launcher<PhysEuler>::stage(N, mode, a, stream);
+      case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(N, mode, nodal, a, stream);
+      case SELFAMD_MODEL_LSW: return Launcher<PhysLSW>::stage(N, mode, nodal, a, stream);
+      default: return Launcher<PhysEuler>::stage(N, mode, nodal, a, stream);
     }
   }
   hipError_t launch_traces() {
@@ -614,35 +657,31 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   SELFAMD_HIP(hipMemcpy(hJ.data(), J, sizeof(double) * nn, hipMemcpyDeviceToHost));
   SELFAMD_HIP(hipMemcpy(hn.data(), nHatb, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost));
   SELFAMD_HIP(hipMemcpy(hs.data(), nScaleb, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  // Per-element affine records: metric terms constant over each element (to round-off, as
+  // the derivative of an interpolated bilinear map leaves ~1e-13 relative noise) use one
+  // record; otherwise the nodal metrics are used.
   std::vector<double> rec((size_t)kGeomStride * nEl, 0.0);
-  for (int e = 0; e < nEl; e++) {
+  bool affine = true;
+  for (int e = 0; e < nEl && affine; e++) {
     double *r = rec.data() + (size_t)kGeomStride * e;
     const double *d0 = hdsdx.data() + 4 * (size_t)PP * e;
     double scale = 0.0;
     for (int k = 0; k < 4; k++) { r[k] = d0[k]; scale = std::fmax(scale, std::fabs(d0[k])); }
     r[4] = hJ[(size_t)PP * e];
-    for (int n = 0; n < PP; n++) {
+    for (int n = 0; n < PP && affine; n++) {
       for (int k = 0; k < 4; k++)
-        if (std::fabs(d0[4 * n + k] - r[k]) > 1e-11 * scale) {
-          set_error("model2d_create: element %d is not affine (dsdx varies); use the operator API", e + 1);
-          return kNotAffine;
-        }
-      if (std::fabs(hJ[(size_t)PP * e + n] - r[4]) > 1e-11 * std::fabs(r[4])) {
-        set_error("model2d_create: element %d is not affine (J varies); use the operator API", e + 1);
-        return kNotAffine;
-      }
+        if (std::fabs(d0[4 * n + k] - r[k]) > 1e-11 * scale) affine = false;
+      if (std::fabs(hJ[(size_t)PP * e + n] - r[4]) > 1e-11 * std::fabs(r[4])) affine = false;
     }
-    for (int s = 0; s < 4; s++) {
+    for (int s = 0; s < 4 && affine; s++) {
       const size_t b0 = (size_t)P * (s + 4 * (size_t)e);
       r[8 + s] = hn[2 * b0];
       r[12 + s] = hn[2 * b0 + 1];
       r[16 + s] = hs[b0];
       for (int i = 1; i < P; i++)
         if (std::fabs(hn[2 * (b0 + i)] - r[8 + s]) > 1e-11 || std::fabs(hn[2 * (b0 + i) + 1] - r[12 + s]) > 1e-11 ||
-            std::fabs(hs[b0 + i] - r[16 + s]) > 1e-11 * std::fabs(r[16 + s])) {
-          set_error("model2d_create: element %d side %d normals vary; use the operator API", e + 1, s + 1);
-          return kNotAffine;
-        }
+            std::fabs(hs[b0 + i] - r[16 + s]) > 1e-11 * std::fabs(r[16 + s]))
+          affine = false;
     }
   }
   // Lagrange matrices on Gauss points
@@ -656,6 +695,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
 
   Model *m = new Model();
   m->kind = kind; m->N = N; m->nEl = nEl;
+  m->affine = affine;
+  m->nodal = !affine;
   m->nVar = selfamd_model2d_nvar(kind);
   m->nT = kind == SELFAMD_MODEL_EULER ? 9 : 3;
   m->p.g = 1.0; m->p.fCori = 0.0; m->p.gamma = 1.4; m->p.Rgas = 1.0;
@@ -677,7 +718,10 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
       hipMalloc(&m->trace[1], sizeof(double) * nT) != hipSuccess ||
       hipMalloc(&m->geom, sizeof(double) * rec.size()) != hipSuccess ||
       hipMalloc(&m->mats, sizeof(double) * mats.size()) != hipSuccess ||
-      hipMalloc(&m->nbr, sizeof(int32_t) * nbr.size()) != hipSuccess) {
+      hipMalloc(&m->nbr, sizeof(int32_t) * nbr.size()) != hipSuccess ||
+      hipMalloc(&m->ndsdx, sizeof(double) * 4 * nn) != hipSuccess || hipMalloc(&m->nJ, sizeof(double) * nn) != hipSuccess ||
+      hipMalloc(&m->nnHat, sizeof(double) * 2 * nb) != hipSuccess ||
+      hipMalloc(&m->nnScale, sizeof(double) * nb) != hipSuccess) {
     set_error("model2d_create: device allocation failed (%zu doubles of state)", nS);
     return fail(kHipError);
   }
@@ -688,6 +732,10 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   if (hipMemcpy(m->geom, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->mats, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->ndsdx, dsdx, sizeof(double) * 4 * nn, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(m->nJ, J, sizeof(double) * nn, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(m->nnHat, nHatb, sizeof(double) * 2 * nb, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(m->nnScale, nScaleb, sizeof(double) * nb, hipMemcpyDeviceToDevice) != hipSuccess ||
       hipMemsetAsync(m->S, 0, sizeof(double) * nS, m->stream) != hipSuccess ||
       hipMemsetAsync(m->G, 0, sizeof(double) * nS, m->stream) != hipSuccess ||
       hipMemsetAsync(m->trace[0], 0, sizeof(double) * nT, m->stream) != hipSuccess ||
@@ -705,7 +753,8 @@ int selfamd_model2d_destroy(selfamd_model2d *m) {
   if (!m) return kOk;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   m->drop_graphs();
-  double *bufs[] = {m->S, m->G, m->trace[0], m->trace[1], m->un, m->geom, m->mats, m->aux, m->stat2, m->presTrace};
+  double *bufs[] = {m->S, m->G, m->trace[0], m->trace[1], m->un, m->geom, m->mats, m->aux, m->stat2, m->presTrace,
+                    m->ndsdx, m->nJ, m->nnHat, m->nnScale};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (m->nbr) (void)hipFree(m->nbr);
@@ -721,6 +770,10 @@ int selfamd_model2d_set_param(selfamd_model2d *m, int which, double value) {
     case SELFAMD_PARAM_FCORI: m->p.fCori = value; break;
     case SELFAMD_PARAM_GAMMA: m->p.gamma = value; break;
     case SELFAMD_PARAM_RGAS: m->p.Rgas = value; break;
+    case SELFAMD_PARAM_NODAL_METRICS:
+      SELFAMD_REQUIRE(value != 0.0 || m->affine, "model2d_set_param: the mesh is not affine; nodal metrics required");
+      m->nodal = value != 0.0;
+      break;
     default: SELFAMD_REQUIRE(false, "model2d_set_param: unknown parameter %d", which);
   }
   m->drop_graphs();
@@ -751,7 +804,9 @@ int selfamd_model2d_set_solution(selfamd_model2d *m, const double *interior) {
   const size_t n = (size_t)(m->N + 1) * (m->N + 1) * m->nVar * m->nEl;
   SELFAMD_HIP(hipMemcpyAsync(m->S, interior, sizeof(double) * n, hipMemcpyDeviceToDevice, m->stream));
   SELFAMD_HIP(m->launch_traces());
-  if (m->kind == SELFAMD_MODEL_SW) SELFAMD_HIP(launch_un_init(m->N, m->trace[m->cur], m->un, m->geom, m->nbr, m->nEl, m->stream));
+  if (m->kind == SELFAMD_MODEL_SW)
+    SELFAMD_HIP(launch_un_init(m->N, m->trace[m->cur], m->un, m->geom, m->nodal ? m->nnHat : nullptr, m->nbr, m->nEl,
+                               m->stream));
   if (m->kind == SELFAMD_MODEL_EULER) {
     // prescribed boundary state := traces of this state (SetPrescribedSolutionFromSolution)
     const size_t nT = (size_t)(m->N + 1) * 9 * 4 * m->nEl;

@@ -24,11 +24,36 @@ def rel_err(a, b, axis_var=1):
     return np.array(out)
 
 
-class Case:
-    """Mesh + oracle geometry + oracle model with a seeded state for one model kind."""
+class DeviceGeometry:
+    """The oracle's geometry arrays uploaded to the device, in SEMQuad's shapes: the GPU engine
+    then sees exactly the metric values the oracle uses (identical inputs)."""
 
-    def __init__(self, kind, N, nx, ny, periodic=False, bcid=self_amd.BC_NONORMALFLOW, seed=1, hopr=None):
-        self.kind, self.N = kind, N
+    def __init__(self, og, N):
+        self.interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
+        for k in ("x", "dsdx", "J", "nHatb", "nScaleb"):
+            setattr(self, k, self_amd.DeviceArray.from_host(np.ascontiguousarray(getattr(og, k))))
+
+
+def flatten_metrics(og):
+    """Metric terms made element-constant (node (0,0) and side node 0), in place: the affine
+    representation the fused engine uses by default on affine meshes."""
+    og.dsdx[...] = og.dsdx[:, :1, :1]
+    og.J[...] = og.J[:, :1, :1]
+    og.nHatb[...] = og.nHatb[:, :, :1]
+    og.nScaleb[...] = og.nScaleb[:, :, :1]
+
+
+class Case:
+    """Mesh + oracle geometry + oracle model with a seeded state for one model kind.
+
+    metrics: "nodal" keeps the per-node metric terms; "affine" flattens them to one value per
+    element on both sides.  generic: adds seeded smooth velocity perturbations so that no
+    variable's tendency vanishes by symmetry (a relative error of a round-off-sized
+    reference value measures nothing)."""
+
+    def __init__(self, kind, N, nx, ny, periodic=False, bcid=self_amd.BC_NONORMALFLOW, seed=1, hopr=None,
+                 metrics="nodal", generic=True):
+        self.kind, self.N, self.metrics = kind, N, metrics
         if hopr is not None:
             self.mesh = hopr
         else:
@@ -36,17 +61,31 @@ class Case:
         lag = oracle.lagrange_init(N, oracle.GAUSS, N, oracle.GAUSS)
         self.nodes, self.weights = lag["controlPoints"], lag["qWeights"]
         self.geom = oracle.Geometry(N, self.mesh.nodeCoords, self.mesh.nGeo)
+        if metrics == "affine":
+            flatten_metrics(self.geom)
         x = self.geom.x
+        rng = np.random.default_rng(seed + 101)
+        smooth = lambda amp: amp * workloads.modal_filter(rng.normal(size=x.shape[:3]), self.nodes, self.weights, 2)
         self.fields = {}
         if kind == "sw":
             self.sol, self.fields["topo"] = workloads.nonlinear_sw(x, self.nodes, self.weights, seed=seed)
         elif kind == "lsw":
             self.sol = workloads.gravity_wave_lsw(x, seed=seed)
-            self.fields["H"] = np.ones(x.shape[:3])
-            self.fields["f"] = np.zeros(x.shape[:3])
+            if generic:
+                self.sol[:, 0] += smooth(0.01)
+                self.sol[:, 1] += smooth(0.01)
+            self.fields["H"] = 1.0 + smooth(0.05) if generic else np.ones(x.shape[:3])
+            self.fields["f"] = smooth(1.0) if generic else np.zeros(x.shape[:3])
         else:
             self.sol, self.fields["phi"] = workloads.euler_hydrostatic(x, seed=seed)
-            self.fields["drag"] = np.zeros(x.shape[:3])
+            if generic:
+                u, v = smooth(0.01), smooth(0.01)
+                rho = self.sol[:, 2]
+                self.sol[:, 0] = rho * u
+                self.sol[:, 1] = rho * v
+                self.sol[:, 3] += 0.5 * rho * (u * u + v * v)
+            self.fields["drag"] = 0.1 + smooth(0.05) if generic else np.zeros(x.shape[:3])
+        self.sol = np.ascontiguousarray(self.sol)
         self.om = oracle.Model(KIND[kind], N, self.mesh.sideInfo, self.geom)
         self.apply_oracle(self.om)
 
@@ -63,9 +102,13 @@ class Case:
         if self.kind == "euler":
             om.set_prescribed()
 
-    def gpu_model(self):
-        interp = self_amd.Lagrange(self.N, self_amd.GAUSS, self.N, self_amd.GAUSS)
-        geom = self_amd.SEMQuad(interp, self.mesh)
+    def gpu_model(self, geometry="oracle"):
+        """GPU model on the same state.  geometry="oracle": the oracle's metric arrays uploaded
+        (identical inputs); "device": metrics generated on the GPU by SEMQuad."""
+        if geometry == "oracle":
+            geom = DeviceGeometry(self.geom, self.N)
+        else:
+            geom = self_amd.SEMQuad(self_amd.Lagrange(self.N, self_amd.GAUSS, self.N, self_amd.GAUSS), self.mesh)
         if self.kind == "sw":
             m = self_amd.ShallowWater(self.mesh, geom)
             m.SetTopography(self.fields["topo"])
@@ -77,16 +120,20 @@ class Case:
             m = self_amd.CompressibleIdealGas2D(self.mesh, geom)
             m.SetGravity(self.fields["phi"])
             m.SetDrag(self.fields["drag"])
+        m.SetNodalMetrics(self.metrics == "nodal")
         m.SetSolution(self.sol)
         return m, geom
 
     def dt(self, cfl=0.5):
         dx = self.geom.covariant_arc_min()
         if self.kind == "euler":
-            c = np.sqrt(1.4 * (self.sol[:, 3] * 0.4) / self.sol[:, 2]).max()
+            rho = self.sol[:, 2]
+            ke = 0.5 * (self.sol[:, 0] ** 2 + self.sol[:, 1] ** 2) / rho
+            p = 0.4 * (self.sol[:, 3] - ke)
+            c = (np.sqrt(1.4 * p / rho) + np.sqrt(2 * ke / rho)).max()
             return workloads.cfl_dt(dx, c, cfl)
         if self.kind == "sw":
             H = self.sol[:, 2]
             speed = (np.sqrt(self.sol[:, 0] ** 2 + self.sol[:, 1] ** 2) / H + np.sqrt(H)).max()
             return workloads.cfl_dt(dx, speed, cfl)
-        return workloads.cfl_dt(dx, 1.0, cfl)
+        return workloads.cfl_dt(dx, np.sqrt(self.fields["H"].max()), cfl)

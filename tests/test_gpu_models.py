@@ -19,15 +19,16 @@ CASES = [
     ("euler", 7, 5, False, self_amd.BC_NONORMALFLOW), ("euler", 4, 4, False, self_amd.BC_PRESCRIBED),
     ("euler", 3, 4, True, 0),
 ]
+PARAMS = [c + (m,) for c in CASES for m in ("nodal", "affine")]
 
 
 def _id(c):
-    return f"{c[0]}-N{c[1]}-{c[2]}x{c[2]}-{'periodic' if c[3] else c[4]}"
+    return f"{c[0]}-N{c[1]}-{c[2]}x{c[2]}-{'periodic' if c[3] else c[4]}-{c[5]}"
 
 
-@pytest.mark.parametrize("kind,N,n,periodic,bcid", CASES, ids=[_id(c) for c in CASES])
-def test_tendency_parity(kind, N, n, periodic, bcid):
-    c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=3)
+@pytest.mark.parametrize("kind,N,n,periodic,bcid,metrics", PARAMS, ids=[_id(c) for c in PARAMS])
+def test_tendency_parity(kind, N, n, periodic, bcid, metrics):
+    c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=3, metrics=metrics)
     m, _ = c.gpu_model()
     d = m.CalculateTendency()
     ref = c.om.tendency()
@@ -38,9 +39,9 @@ def test_tendency_parity(kind, N, n, periodic, bcid):
     assert rel_err(d2, c.om.tendency()).max() <= TOL
 
 
-@pytest.mark.parametrize("kind,N,n,periodic,bcid", CASES, ids=[_id(c) for c in CASES])
-def test_rk3_trajectory_parity(kind, N, n, periodic, bcid):
-    c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=4)
+@pytest.mark.parametrize("kind,N,n,periodic,bcid,metrics", PARAMS, ids=[_id(c) for c in PARAMS])
+def test_rk3_trajectory_parity(kind, N, n, periodic, bcid, metrics):
+    c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=4, metrics=metrics)
     m, _ = c.gpu_model()
     dt = c.dt()
     for chunk in (3, 7):  # two ForwardStep calls: the engine's state persists between them
@@ -51,8 +52,20 @@ def test_rk3_trajectory_parity(kind, N, n, periodic, bcid):
     assert abs(m.t - c.om.get_scalar("t")) <= 1e-12 * abs(m.t)
 
 
+@pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
+def test_end_to_end_device_geometry(kind):
+    """Mesh -> device-generated metrics -> nodal-metric engine, against the oracle's own
+    geometry: the whole reference pipeline on identical mesh and state inputs."""
+    c = Case(kind, 7, 5, 5, periodic=(kind == "sw"), seed=6)
+    m, _ = c.gpu_model(geometry="device")
+    assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
+    m.ForwardStep(4, c.dt())
+    c.om.rk3_steps(4, c.dt())
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+
+
 def test_tendency_then_steps_keeps_lag_state():
-    c = Case("sw", 7, 4, 4, periodic=True, seed=9)
+    c = Case("sw", 7, 4, 4, periodic=True, seed=9, metrics="affine")
     m, _ = c.gpu_model()
     m.CalculateTendency()
     c.om.tendency()
@@ -72,7 +85,7 @@ def test_hopr_block_mesh_parity():
         np.array(fx["NodeCoords"]["data"]).reshape(fx["NodeCoords"]["shape"]) / 1.0e6 + 0.5,
         np.array(fx["GlobalNodeIDs"]["data"], np.int32))
     mesh.ResetBoundaryConditionType(self_amd.BC_NONORMALFLOW)
-    c = Case("lsw", 7, 0, 0, hopr=mesh, seed=2)
+    c = Case("lsw", 7, 0, 0, hopr=mesh, seed=2, metrics="affine")
     m, _ = c.gpu_model()
     assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
     m.ForwardStep(5, c.dt())
@@ -81,10 +94,11 @@ def test_hopr_block_mesh_parity():
 
 
 def test_full_size_sw_256():
-    """Headline configuration (256^2 elements, N = 7, periodic): two RK3 steps against the
-    oracle run with all host threads, then mass conservation over 20 more steps."""
+    """Headline configuration (256^2 elements, N = 7, periodic, affine metrics as benchmarked):
+    two RK3 steps against the oracle run with all host threads, then mass conservation over
+    20 more steps."""
     import os
-    c = Case("sw", 7, 256, 256, periodic=True, seed=0)
+    c = Case("sw", 7, 256, 256, periodic=True, seed=0, metrics="affine")
     m, geom = c.gpu_model()
     dt = c.dt()
     oracle.set_threads(min(16, os.cpu_count() or 1))
@@ -95,7 +109,7 @@ def test_full_size_sw_256():
     m.ForwardStep(2, dt)
     assert rel_err(m.solution, c.om.solution).max() <= TOL
     w = c.weights
-    J = geom.J.to_host()
+    J = c.geom.J
     def mass(s):
         return float(np.sum(s[:, 2] * J * w[None, :, None] * w[None, None, :]))
     m0 = mass(c.sol)

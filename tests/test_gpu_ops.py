@@ -172,9 +172,11 @@ def test_mapped_operators(rng):
 
 
 def _physics_inputs(kind, bcid, periodic=False):
-    """Oracle model after one CalculateTendency: every register of the stage is populated."""
+    """Oracle model after one CalculateTendency, with the physical flux recomputed (the
+    tendency projects it in place): every register of the stage is populated."""
     c = Case(kind, 5, 4, 4, periodic=periodic, bcid=bcid, seed=5)
     oracle.lib().or_calculate_tendency(c.om._p)
+    oracle.lib().or_flux(c.om._p)
     return c
 
 

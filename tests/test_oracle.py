@@ -267,7 +267,7 @@ def test_sw_quiet_fluid():
 
 @pytest.mark.parametrize("kind", ["lsw", "euler"])
 def test_free_stream_preservation(kind):
-    c = Case(kind, 4, 3, 3, periodic=True)
+    c = Case(kind, 4, 3, 3, periodic=True, generic=False)
     nel = c.mesh.nElem
     sol = np.zeros((nel, c.om.nVar, 5, 5))
     if kind == "lsw":
