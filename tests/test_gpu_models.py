@@ -43,7 +43,7 @@ def test_tendency_parity(kind, N, n, periodic, bcid, metrics):
 def test_rk3_trajectory_parity(kind, N, n, periodic, bcid, metrics):
     c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=4, metrics=metrics)
     m, _ = c.gpu_model()
-    dt = c.dt()
+    dt = c.dt(cfl=0.5 if N >= 3 else 0.05)  # CovariantArcMin scales as 1/N^2: too large a step at N <= 2
     for chunk in (3, 7):  # two ForwardStep calls: the engine's state persists between them
         m.ForwardStep(chunk, dt)
         c.om.rk3_steps(chunk, dt)
@@ -55,9 +55,13 @@ def test_rk3_trajectory_parity(kind, N, n, periodic, bcid, metrics):
 @pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
 def test_end_to_end_device_geometry(kind):
     """Mesh -> device-generated metrics -> nodal-metric engine, against the oracle's own
-    geometry: the whole reference pipeline on identical mesh and state inputs."""
+    geometry: the whole reference pipeline on identical mesh and state inputs.  The device
+    geometry is bit-identical to the oracle's, which this relies on: the weak divergence
+    amplifies metric round-off roughly a hundredfold."""
     c = Case(kind, 7, 5, 5, periodic=(kind == "sw"), seed=6)
-    m, _ = c.gpu_model(geometry="device")
+    m, geom = c.gpu_model(geometry="device")
+    for k in ("x", "dsdx", "J", "nHatb", "nScaleb"):
+        np.testing.assert_array_equal(getattr(geom, k).to_host(), getattr(c.geom, k), err_msg=k)
     assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
     m.ForwardStep(4, c.dt())
     c.om.rk3_steps(4, c.dt())
