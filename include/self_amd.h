@@ -68,6 +68,11 @@ int selfamd_free(void *ptr);
 int selfamd_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int selfamd_memset(void *ptr, int value, size_t bytes);
 int selfamd_device_synchronize(void);
+/* dst = src for n doubles (n even, 16-byte aligned pointers): a streaming kernel of known
+ * traffic (16 n bytes) for bandwidth and counter calibration.  No reference counterpart. */
+int selfamd_copy_f64(const double *src_dev, double *dst_dev, size_t n, void *stream);
+/* hipEvent-timed repetitions of selfamd_copy_f64 on `stream`; returns total milliseconds. */
+int selfamd_time_copy_f64(const double *src_dev, double *dst_dev, size_t n, int reps, double *elapsedMs);
 
 /* ------------------------------------------------------------------------------------ */
 /* Host setup                                                                            */
@@ -290,6 +295,23 @@ int selfamd_model2d_get_time(selfamd_model2d *model, double *t);
 int selfamd_model2d_set_time(selfamd_model2d *model, double t);
 /* Number of conserved variables (3 for LSW/SW, 4 for Euler). */
 int selfamd_model2d_nvar(int kind);
+
+/* ------------------------------------------------------------------------------------ */
+/* Batched runs (configuration C4): many independent models advanced together            */
+/* ------------------------------------------------------------------------------------ */
+/* No reference counterpart (the reference runs one model per process).  Models of the same
+ * kind, affine metrics and identical physical parameters; runs of one polynomial degree are
+ * concatenated into one mesh and advanced by one fused launch per RK stage, all degree groups
+ * of a step in one hipGraph.  dt[r] is run r's step.  Creation copies the runs' current
+ * state; selfamd_batch_scatter copies it back, after which each model continues alone
+ * (bit-identical to having been advanced alone). */
+typedef struct selfamd_batch selfamd_batch;
+int selfamd_batch_create(selfamd_batch **batch, int nModels, selfamd_model2d *const *models, const double *dt);
+int selfamd_batch_forward_step(selfamd_batch *batch, int nSteps);
+int selfamd_batch_time_steps(selfamd_batch *batch, int nSteps, double *elapsedMs);
+int selfamd_batch_scatter(selfamd_batch *batch);
+int selfamd_batch_info(selfamd_batch *batch, int *nGroups, double *totalDof);
+int selfamd_batch_destroy(selfamd_batch *batch);
 
 #ifdef __cplusplus
 }

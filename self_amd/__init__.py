@@ -25,7 +25,7 @@ PARAM_G, PARAM_FCORI, PARAM_GAMMA, PARAM_RGAS, PARAM_NODAL_METRICS = 1, 2, 3, 4,
 FIELD_TOPOGRAPHY, FIELD_BATHYMETRY, FIELD_CORIOLIS, FIELD_POTENTIAL, FIELD_DRAG = 1, 2, 3, 4, 5
 
 __all__ = ["DeviceArray", "device_count", "device_name", "synchronize", "Lagrange", "Mesh2D", "SEMQuad",
-           "Model2D", "ShallowWater", "LinearShallowWater", "CompressibleIdealGas2D", "SelfAmdError", "load", "GAUSS", "GAUSS_LOBATTO", "UNIFORM",
+           "Model2D", "ShallowWater", "LinearShallowWater", "CompressibleIdealGas2D", "Batch", "SelfAmdError", "load", "GAUSS", "GAUSS_LOBATTO", "UNIFORM",
            "BC_PRESCRIBED", "BC_RADIATION", "BC_NONORMALFLOW"]
 
 
@@ -363,3 +363,43 @@ class CompressibleIdealGas2D(Model2D):
         if np.isscalar(cd):
             cd = np.full((self.nElem, self.N + 1, self.N + 1), float(cd))
         self._field(FIELD_DRAG, cd)
+
+
+class Batch:
+    """Many independent models advanced together (no reference counterpart; configuration C4).
+
+    Runs of one polynomial degree share one fused launch per RK stage and every degree group
+    of a step sits in one hipGraph.  `scatter()` hands each run's state back to its model."""
+
+    def __init__(self, models, dts):
+        assert len(models) == len(dts) and models
+        self.models = list(models)
+        arr = (ctypes.c_void_p * len(models))(*[m._h.value for m in models])
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        self._h = ctypes.c_void_p()
+        call("selfamd_batch_create", ctypes.byref(self._h), len(models), arr, _ptr(dt))
+
+    def ForwardStep(self, nSteps):
+        call("selfamd_batch_forward_step", self._h, int(nSteps))
+
+    def time_steps(self, nSteps):
+        ms = ctypes.c_double()
+        call("selfamd_batch_time_steps", self._h, int(nSteps), ctypes.byref(ms))
+        return ms.value
+
+    def scatter(self):
+        call("selfamd_batch_scatter", self._h)
+
+    def info(self):
+        g, dof = ctypes.c_int(), ctypes.c_double()
+        call("selfamd_batch_info", self._h, ctypes.byref(g), ctypes.byref(dof))
+        return g.value, dof.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().selfamd_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None

@@ -27,6 +27,8 @@ SIGNATURES = {
     "selfamd_memcpy": [c_void_p, c_void_p, c_size_t, c_int],
     "selfamd_memset": [c_void_p, c_int, c_size_t],
     "selfamd_device_synchronize": [],
+    "selfamd_copy_f64": [c_void_p, c_void_p, c_size_t, c_void_p],
+    "selfamd_time_copy_f64": [c_void_p, c_void_p, c_size_t, c_int, c_void_p],
     "selfamd_version": [],
     "selfamd_lagrange_init": [c_int, c_int, c_int, c_int] + [c_void_p] * 8,
     "selfamd_mesh2d_structured": [c_int, c_int, c_double, c_double, c_double, c_double, c_int, c_int, c_int,
@@ -80,6 +82,12 @@ SIGNATURES = {
     "selfamd_model2d_time_steps": [c_void_p, c_int, c_double, c_void_p],
     "selfamd_model2d_get_time": [c_void_p, c_void_p],
     "selfamd_model2d_set_time": [c_void_p, c_double],
+    "selfamd_batch_create": [c_void_p, c_int, c_void_p, c_void_p],
+    "selfamd_batch_forward_step": [c_void_p, c_int],
+    "selfamd_batch_time_steps": [c_void_p, c_int, c_void_p],
+    "selfamd_batch_scatter": [c_void_p],
+    "selfamd_batch_info": [c_void_p, c_void_p, c_void_p],
+    "selfamd_batch_destroy": [c_void_p],
 }
 
 

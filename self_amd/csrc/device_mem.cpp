@@ -52,6 +52,29 @@ int selfamd_memset(void *ptr, int value, size_t bytes) {
   return kOk;
 }
 
+int selfamd_time_copy_f64(const double *src, double *dst, size_t n, int reps, double *elapsedMs) {
+  SELFAMD_REQUIRE(elapsedMs && reps > 0, "time_copy_f64: bad arguments");
+  hipStream_t st;
+  SELFAMD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  SELFAMD_HIP(hipEventCreate(&e0));
+  SELFAMD_HIP(hipEventCreate(&e1));
+  int rc = selfamd_copy_f64(src, dst, n, st);  // warm-up
+  if (rc) return rc;
+  SELFAMD_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps && rc == 0; r++) rc = selfamd_copy_f64(src, dst, n, st);
+  if (rc) return rc;
+  SELFAMD_HIP(hipEventRecord(e1, st));
+  SELFAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SELFAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *elapsedMs = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SELFAMD_HIP(hipStreamDestroy(st));
+  return kOk;
+}
+
 int selfamd_device_synchronize(void) {
   SELFAMD_HIP(hipDeviceSynchronize());
   return kOk;

@@ -32,7 +32,7 @@ using namespace selfamd;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGeomStride = 20;  // dsdx(4) J(1) pad(3) nx(4) ny(4) nScale(4)
+constexpr int kGeomStride = 20;  // dsdx(4) J(1) dt(1, batched runs) pad(2) nx(4) ny(4) nScale(4)
 constexpr int kNbrOrphan = -0x40000000;
 
 // Williamson low-storage RK3 (src/SELF_Model.f90:25-27)
@@ -58,6 +58,8 @@ struct FusedArgs {
   const double *mats;       // dgM(P*P) D(P*P) bM(2P) qW(P)
   int nEl;
   double rkA, rkGdt;        // stage coefficients a_m and g_m*dt
+  double rkG;               // g_m alone: batched runs take dt from their element records
+  int perElemDt;            // 1: g_m * dt_e with dt_e in the element record (batched runs)
   Params p;
 };
 
@@ -233,12 +235,21 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
 // MODE 0: RK stage (update S, G, write next traces); MODE 1: tendency only (write dSdt).
 // NODAL: metric terms per node and per face node exactly as SEMQuad stores them (curved
 // elements, and the reference's own per-node metric values); otherwise one affine record
-// per element.
-template <class Phys, int N, int MODE, bool NODAL>
-__global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
-  constexpr int NP = N + 1, NN = NP * NP, EPB = kThreads / NN, NF = 4 * NP;
+// per element.  THREADS: block size (the tile shape); EPB = THREADS/(N+1)^2 elements per
+// block.  With THREADS = 64 at N = 7 a block is one wavefront owning one element, so its
+// barriers never wait on other waves.
+//
+// Every global load the stage needs is issued before any arithmetic: the element's state,
+// RK register and static field on the node lanes, and (on the face lanes) the neighbour
+// code, the neighbour's traces and the lagged normal velocities.  The gather of neighbour
+// traces is the only dependent load and it overlaps the volume-flux work.
+template <class Phys, int N, int MODE, bool NODAL, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
+  constexpr int NP = N + 1, NN = NP * NP, EPB = THREADS / NN, NF = 4 * NP;
   constexpr int NV = Phys::NV, NT = Phys::NT;
   constexpr bool USE_AUX = Phys::kAuxGrad || Phys::kAuxTrace;
+  constexpr bool ONE_FACE_PASS = EPB * NF <= THREADS;  // every face node has its own lane
+  static_assert(EPB >= 1, "block too small for one element");
   __shared__ double sQ[NT][EPB][NN];
   __shared__ double sF1[NV][EPB][NN];
   __shared__ double sF2[NV][EPB][NN];
@@ -250,28 +261,17 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   __shared__ double sM[2 * NN + 3 * NP];  // dgM, D, bM(2P), qW
 
   const int tid = threadIdx.x;
-  for (int k = tid; k < 2 * NN + 3 * NP; k += kThreads) sM[k] = a.mats[k];
-  const double *dgM = sM, *Dm = sM + NN, *bM0 = sM + 2 * NN, *bM1 = bM0 + NP, *qW = bM1 + NP;
-
   const int el = tid / NN, node = tid % NN;
   const int i = node % NP, j = node / NP;
   const int e = blockIdx.x * EPB + el;
   const bool nodeActive = el < EPB && e < a.nEl;
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
 
+  // ---- loads: node lanes ----
   double s[NV], gReg[NV];
-  double statVal = a.p.stat2Const;
+  double auxVal = a.p.auxConst, statVal = a.p.stat2Const;
   double dsdx[4] = {0, 0, 0, 0}, jac = 1.0;
   if (nodeActive) {
-    if constexpr (NODAL) {
-      const double *gn = a.ndsdx + 4 * ((size_t)e * NN + node);
-      dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
-      jac = a.nJ[(size_t)e * NN + node];
-    } else {
-      const double *gr = a.geom + (size_t)e * kGeomStride;
-      dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
-      jac = gr[4];
-    }
 #pragma unroll
     for (int v = 0; v < NV; v++) s[v] = a.S[sBase + (size_t)v * NN + node];
     if (MODE == 0 && a.rkA != 0.0) {
@@ -281,9 +281,53 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
 #pragma unroll
       for (int v = 0; v < NV; v++) gReg[v] = 0.0;
     }
-    double auxVal = a.p.auxConst;
     if (USE_AUX && a.aux) auxVal = a.aux[(size_t)e * NN + node];
     if (a.stat2) statVal = a.stat2[(size_t)e * NN + node];
+    if constexpr (NODAL) {
+      const double *gn = a.ndsdx + 4 * ((size_t)e * NN + node);
+      dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
+      jac = a.nJ[(size_t)e * NN + node];
+    } else {
+      const double *gr = a.geom + (size_t)e * kGeomStride;
+      dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
+      jac = gr[4];
+    }
+  }
+
+  // ---- loads: face lanes (neighbour traces, lagged velocities, normals) ----
+  const int fel = tid / NF, fnode = tid % NF;
+  const int side = fnode / NP, fi = fnode % NP;
+  const int fe = blockIdx.x * EPB + fel;
+  const bool faceActive = ONE_FACE_PASS && fel < EPB && fe < a.nEl;
+  double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
+  int code = 0;
+  if (faceActive) {
+    code = a.nbr[4 * (size_t)fe + side];
+    if (code >= 0) {
+      const int flip = code & 1, s2 = (code >> 1) & 3;
+      const size_t e2 = (size_t)(code >> 3);
+      const double *tr = a.traceIn + (size_t)(flip ? N - fi : fi) + NP * (size_t)NT * (s2 + 4 * e2);
+#pragma unroll
+      for (int k = 0; k < NT; k++) qR[k] = tr[(size_t)k * NP];
+    }
+    if (Phys::kLag) {
+      const double *un = a.un + (size_t)fi + NP * 2 * (side + 4 * (size_t)fe);
+      unL = un[0];
+      unR = un[NP];
+    }
+    if constexpr (NODAL) {
+      const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
+      nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
+    } else {
+      const double *gr = a.geom + (size_t)fe * kGeomStride;
+      nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
+    }
+  }
+  for (int k = tid; k < 2 * NN + 3 * NP; k += THREADS) sM[k] = a.mats[k];
+  const double *dgM = sM, *Dm = sM + NN, *bM0 = sM + 2 * NN, *bM1 = bM0 + NP, *qW = bM1 + NP;
+
+  // ---- phase 0: volume fluxes, nodal trace quantities ----
+  if (nodeActive) {
     double fx[NV], fy[NV], q[NT];
     Phys::volume(s, auxVal, a.p, fx, fy, q);
 #pragma unroll
@@ -302,70 +346,81 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   }
   __syncthreads();
 
-  // ---- phase 1: faces (one pass for N >= 3; N = 1, 2 have more face nodes than lanes) ----
-  for (int ft = tid; ft < EPB * NF; ft += kThreads) {
-    const int fel = ft / NF, fnode = ft % NF;
-    const int side = fnode / NP, fi = fnode % NP;
-    const int fe = blockIdx.x * EPB + fel;
-    if (fe >= a.nEl) continue;
+  // ---- phase 1: faces ----
+  auto face = [&](int fel_, int side_, int fi_, int fe_, int fnode_, int code_, double *qR_, double unL_, double unR_,
+                  double nx_, double ny_, double nmag_) {
     // own traces, ScalarBoundaryInterp_2D order (src/SELF_Lagrange.f90:2241-2270)
-    const double *bm = (side == 1 || side == 2) ? bM1 : bM0;
-    const bool alongJ = (side == 0 || side == 2);
-    double qL[NT], qR[NT];
+    const double *bm = (side_ == 1 || side_ == 2) ? bM1 : bM0;
+    const bool alongJ = (side_ == 0 || side_ == 2);
+    double qL[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qL[k] = 0.0;
     double auxB = a.p.auxConst;
     if (Phys::kAuxTrace && a.aux) auxB = 0.0;
     for (int ii = 0; ii < NP; ii++) {
-      const int nd = alongJ ? fi + NP * ii : ii + NP * fi;
+      const int nd = alongJ ? fi_ + NP * ii : ii + NP * fi_;
 #pragma unroll
-      for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel][nd];
-      if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel][nd];
+      for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
+      if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
     }
-    double nx, ny, nmag;
-    if constexpr (NODAL) {
-      const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
-      nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
-    } else {
-      const double *gr = a.geom + (size_t)fe * kGeomStride;
-      nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
-    }
-    const int code = a.nbr[4 * (size_t)fe + side];
-    bool interior = false;
-    if (code >= 0) {
-      interior = true;
-      const int flip = code & 1, s2 = (code >> 1) & 3;
-      const size_t e2 = (size_t)(code >> 3);
-      const int i2 = flip ? N - fi : fi;
-      const double *tr = a.traceIn + (size_t)i2 + NP * (size_t)NT * (s2 + 4 * e2);
+    const bool interior = code_ >= 0;
+    if (!interior) {
+      if (code_ == kNbrOrphan) {
 #pragma unroll
-      for (int k = 0; k < NT; k++) qR[k] = tr[(size_t)k * NP];
-    } else if (code == kNbrOrphan) {
+        for (int k = 0; k < NT; k++) qR_[k] = 0.0;
+      } else {
+        const double *pres =
+            a.presTrace ? a.presTrace + (size_t)fi_ + NP * (size_t)NT * (side_ + 4 * (size_t)fe_) : nullptr;
+        double pv[NT];
+        if (pres) {
 #pragma unroll
-      for (int k = 0; k < NT; k++) qR[k] = 0.0;
-    } else {
-      const double *pres = a.presTrace ? a.presTrace + (size_t)fi + NP * (size_t)NT * (side + 4 * (size_t)fe) : nullptr;
-      double pv[NT];
-      if (pres) {
-#pragma unroll
-        for (int k = 0; k < NT; k++) pv[k] = pres[(size_t)k * NP];
+          for (int k = 0; k < NT; k++) pv[k] = pres[(size_t)k * NP];
+        }
+        Phys::bc(qL, auxB, nx_, ny_, -code_ - 1, pres ? pv : nullptr, qR_);
       }
-      Phys::bc(qL, auxB, nx, ny, -code - 1, pres ? pv : nullptr, qR);
     }
-    double unL = 0.0, unR = 0.0;
     if (Phys::kLag) {
-      // velocity traces lag one stage (src/models/SELF_ShallowWater.f90:222-237): use the
-      // stored normal velocities, then store this stage's for the next one.
-      double *un = a.un + (size_t)fi + NP * 2 * (side + 4 * (size_t)fe);
-      unL = un[0];
-      unR = un[NP];
-      un[0] = (qL[0] / qL[2]) * nx + (qL[1] / qL[2]) * ny;
-      un[NP] = interior ? (qR[0] / qR[2]) * nx + (qR[1] / qR[2]) * ny : 0.0;
+      // velocity traces lag one stage (src/models/SELF_ShallowWater.f90:222-237): the stored
+      // normal velocities are used now and this stage's are stored for the next one.
+      double *un = a.un + (size_t)fi_ + NP * 2 * (side_ + 4 * (size_t)fe_);
+      un[0] = (qL[0] / qL[2]) * nx_ + (qL[1] / qL[2]) * ny_;
+      un[NP] = interior ? (qR_[0] / qR_[2]) * nx_ + (qR_[1] / qR_[2]) * ny_ : 0.0;
     }
     double fstar[NV];
-    Phys::riemann(qL, qR, nx, ny, nmag, auxB, unL, unR, a.p, fstar);
+    Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, unR_, a.p, fstar);
 #pragma unroll
-    for (int v = 0; v < NV; v++) sBF[v][fel][fnode] = fstar[v];
+    for (int v = 0; v < NV; v++) sBF[v][fel_][fnode_] = fstar[v];
+  };
+  if constexpr (ONE_FACE_PASS) {
+    if (faceActive) face(fel, side, fi, fe, fnode, code, qR, unL, unR, nx, ny, nmag);
+  } else {
+    // N = 1, 2: more face nodes than lanes; load and compute face by face
+    for (int ft = tid; ft < EPB * NF; ft += THREADS) {
+      const int fel2 = ft / NF, fnode2 = ft % NF, side2 = fnode2 / NP, fi2 = fnode2 % NP;
+      const int fe2 = blockIdx.x * EPB + fel2;
+      if (fe2 >= a.nEl) continue;
+      double qR2[NT], unL2 = 0.0, unR2 = 0.0, nx2, ny2, nmag2;
+      const int code2 = a.nbr[4 * (size_t)fe2 + side2];
+      if (code2 >= 0) {
+        const int flip = code2 & 1, s2 = (code2 >> 1) & 3;
+        const size_t e2 = (size_t)(code2 >> 3);
+        const double *tr = a.traceIn + (size_t)(flip ? N - fi2 : fi2) + NP * (size_t)NT * (s2 + 4 * e2);
+        for (int k = 0; k < NT; k++) qR2[k] = tr[(size_t)k * NP];
+      }
+      if (Phys::kLag) {
+        const double *un = a.un + (size_t)fi2 + NP * 2 * (side2 + 4 * (size_t)fe2);
+        unL2 = un[0];
+        unR2 = un[NP];
+      }
+      if constexpr (NODAL) {
+        const size_t fb = (size_t)fi2 + NP * (side2 + 4 * (size_t)fe2);
+        nx2 = a.nnHat[2 * fb]; ny2 = a.nnHat[2 * fb + 1]; nmag2 = a.nnScale[fb];
+      } else {
+        const double *gr = a.geom + (size_t)fe2 * kGeomStride;
+        nx2 = gr[8 + side2]; ny2 = gr[12 + side2]; nmag2 = gr[16 + side2];
+      }
+      face(fel2, side2, fi2, fe2, fnode2, code2, qR2, unL2, unR2, nx2, ny2, nmag2);
+    }
   }
   __syncthreads();
 
@@ -389,6 +444,8 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
     }
     double src[NV];
     Phys::source(s, gx, gy, statVal, a.p, src);
+    // per-run step size of a batched launch: the same IEEE product g_m*dt as a single run
+    const double gdt = (MODE == 0 && a.perElemDt) ? a.rkG * a.geom[(size_t)e * kGeomStride + 5] : a.rkGdt;
     const double wi = qW[i], wj = qW[j];
     const double bi1 = bM1[i], bi0 = bM0[i], bj1 = bM1[j], bj0 = bM0[j];
 #pragma unroll
@@ -405,7 +462,7 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
         a.dSdt[sBase + (size_t)v * NN + node] = dsdt;
       } else {
         const double gn = a.rkA * gReg[v] + dsdt;
-        s[v] = s[v] + a.rkGdt * gn;
+        s[v] = s[v] + gdt * gn;
         a.G[sBase + (size_t)v * NN + node] = gn;
         a.S[sBase + (size_t)v * NN + node] = s[v];
       }
@@ -421,22 +478,22 @@ __global__ void __launch_bounds__(kThreads) k_fused(const FusedArgs a) {
   __syncthreads();
 
   // ---- phase 3: traces of the updated state ----
-  for (int ft = tid; ft < EPB * NF; ft += kThreads) {
-    const int fel = ft / NF, fnode = ft % NF;
-    const int side = fnode / NP, fi = fnode % NP;
-    const int fe = blockIdx.x * EPB + fel;
-    if (fe >= a.nEl) continue;
-    const double *bm = (side == 1 || side == 2) ? bM1 : bM0;
-    const bool alongJ = (side == 0 || side == 2);
+  for (int ft = tid; ft < EPB * NF; ft += THREADS) {
+    const int fel3 = ft / NF, fnode3 = ft % NF;
+    const int side3 = fnode3 / NP, fi3 = fnode3 % NP;
+    const int fe3 = blockIdx.x * EPB + fel3;
+    if (fe3 >= a.nEl) continue;
+    const double *bm = (side3 == 1 || side3 == 2) ? bM1 : bM0;
+    const bool alongJ = (side3 == 0 || side3 == 2);
     double qn[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qn[k] = 0.0;
     for (int ii = 0; ii < NP; ii++) {
-      const int nd = alongJ ? fi + NP * ii : ii + NP * fi;
+      const int nd = alongJ ? fi3 + NP * ii : ii + NP * fi3;
 #pragma unroll
-      for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel][nd];
+      for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel3][nd];
     }
-    double *tr = a.traceOut + (size_t)fi + NP * (size_t)NT * (side + 4 * (size_t)fe);
+    double *tr = a.traceOut + (size_t)fi3 + NP * (size_t)NT * (side3 + 4 * (size_t)fe3);
 #pragma unroll
     for (int k = 0; k < NT; k++) tr[(size_t)k * NP] = qn[k];
   }
@@ -508,22 +565,30 @@ __global__ void k_sw_un_init(const double *trace, double *un, const double *geom
   }
 }
 
+// Block size (tile) per polynomial degree: one wavefront per element at N = 7, larger
+// blocks where an element covers only part of a wavefront.
+template <int N>
+constexpr int tile_threads() {
+  return N == 7 ? 64 : N == 6 ? 64 : N == 5 ? 128 : N == 4 ? 128 : 256;
+}
+
 template <class Phys>
 struct Launcher {
   static hipError_t stage(int N, int mode, bool nodal, const FusedArgs &a, hipStream_t st) {
     switch (N) {
-#define CASE(NN_)                                                                                   \
-  case NN_: {                                                                                       \
-    constexpr int EPB = kThreads / ((NN_ + 1) * (NN_ + 1));                                         \
-    dim3 grid((a.nEl + EPB - 1) / EPB);                                                             \
-    if (nodal) {                                                                                    \
-      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, true>), grid, kThreads, 0, st, a);   \
-      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, true>), grid, kThreads, 0, st, a);             \
-    } else {                                                                                        \
-      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, false>), grid, kThreads, 0, st, a);  \
-      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, false>), grid, kThreads, 0, st, a);            \
-    }                                                                                               \
-    return hipGetLastError();                                                                       \
+#define CASE(NN_)                                                                                     \
+  case NN_: {                                                                                         \
+    constexpr int T = tile_threads<NN_>();                                                            \
+    constexpr int EPB = T / ((NN_ + 1) * (NN_ + 1));                                                  \
+    dim3 grid((a.nEl + EPB - 1) / EPB);                                                               \
+    if (nodal) {                                                                                      \
+      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, true, T>), grid, T, 0, st, a);         \
+      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, true, T>), grid, T, 0, st, a);                   \
+    } else {                                                                                          \
+      if (mode == 0) hipLaunchKernelGGL((k_fused<Phys, NN_, 0, false, T>), grid, T, 0, st, a);        \
+      else hipLaunchKernelGGL((k_fused<Phys, NN_, 1, false, T>), grid, T, 0, st, a);                  \
+    }                                                                                                 \
+    return hipGetLastError();                                                                         \
   }
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
@@ -591,6 +656,8 @@ struct selfamd_model2d {
     a.ndsdx = ndsdx; a.nJ = nJ; a.nnHat = nnHat; a.nnScale = nnScale;
     a.rkA = kRk3A[stage];
     a.rkGdt = kRk3G[stage] * dt;
+    a.rkG = kRk3G[stage];
+    a.perElemDt = 0;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(N, mode, nodal, a, stream);
@@ -896,6 +963,227 @@ int selfamd_model2d_get_time(selfamd_model2d *m, double *t) {
 int selfamd_model2d_set_time(selfamd_model2d *m, double t) {
   SELFAMD_REQUIRE(m, "model2d_set_time: null model");
   m->t = t;
+  return kOk;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------
+// Batched runs: many independent models advanced together
+// ------------------------------------------------------------------------------
+// Models of one polynomial degree are concatenated into one virtual mesh (state, traces,
+// static fields, neighbour codes offset by each run's first element); each element record
+// carries its run's dt.  One fused launch per degree group and RK stage advances every run
+// of the group, and one hipGraph holds all groups' three stages of a step.  Per element the
+// arithmetic is that of the single-run engine, so a batched run is bit-identical to the same
+// run advanced alone.
+namespace {
+struct BatchGroup {
+  int N = 0, nEl = 0;
+  double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un = nullptr, *geom = nullptr;
+  double *mats = nullptr, *aux = nullptr, *stat2 = nullptr, *presTrace = nullptr;
+  int32_t *nbr = nullptr;
+  std::vector<int> runs;
+  std::vector<size_t> first;  // first element of each run in the group
+};
+}  // namespace
+
+struct selfamd_batch {
+  int kind = 0;
+  std::vector<Model *> models;
+  std::vector<double> dt;
+  std::vector<BatchGroup> groups;
+  Params p{};
+  hipStream_t stream = nullptr;
+  int cur = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+  hipError_t launch_group(BatchGroup &g, int stage, int tin) {
+    FusedArgs a{};
+    a.S = g.S; a.G = g.G; a.dSdt = nullptr;
+    a.traceIn = g.trace[tin]; a.traceOut = g.trace[tin ^ 1];
+    a.un = g.un; a.aux = g.aux; a.stat2 = g.stat2; a.presTrace = g.presTrace;
+    a.geom = g.geom; a.nbr = g.nbr; a.mats = g.mats; a.nEl = g.nEl;
+    a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
+    a.p = p;
+    switch (kind) {
+      case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);
+      case SELFAMD_MODEL_LSW: return Launcher<PhysLSW>::stage(g.N, 0, false, a, stream);
+      default: return Launcher<PhysEuler>::stage(g.N, 0, false, a, stream);
+    }
+  }
+};
+
+extern "C" {
+
+int selfamd_batch_destroy(selfamd_batch *b) {
+  if (!b) return kOk;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &gx : b->graph)
+    if (gx) (void)hipGraphExecDestroy(gx);
+  for (auto &g : b->groups) {
+    double *bufs[] = {g.S, g.G, g.trace[0], g.trace[1], g.un, g.geom, g.mats, g.aux, g.stat2, g.presTrace};
+    for (double *p : bufs)
+      if (p) (void)hipFree(p);
+    if (g.nbr) (void)hipFree(g.nbr);
+  }
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return kOk;
+}
+
+int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *const *models, const double *dt) {
+  SELFAMD_REQUIRE(out && models && dt && nModels >= 1, "batch_create: bad arguments");
+  *out = nullptr;
+  const Model *m0 = models[0];
+  SELFAMD_REQUIRE(m0, "batch_create: null model");
+  for (int r = 0; r < nModels; r++) {
+    const Model *m = models[r];
+    SELFAMD_REQUIRE(m && m->kind == m0->kind, "batch_create: run %d: all runs must be the same model kind", r);
+    SELFAMD_REQUIRE(!m->nodal, "batch_create: run %d uses nodal metrics; batched runs take affine meshes", r);
+    SELFAMD_REQUIRE(dt[r] > 0.0, "batch_create: run %d: dt must be positive", r);
+    SELFAMD_REQUIRE(memcmp(&m->p, &m0->p, sizeof(Params)) == 0,
+                    "batch_create: run %d: physical parameters differ from run 0", r);
+    SELFAMD_REQUIRE((m->aux != nullptr) == (m0->aux != nullptr) && (m->stat2 != nullptr) == (m0->stat2 != nullptr) &&
+                        (m->presTrace != nullptr) == (m0->presTrace != nullptr),
+                    "batch_create: run %d: static fields must be set on all runs or on none", r);
+  }
+  selfamd_batch *b = new selfamd_batch();
+  b->kind = m0->kind;
+  b->p = m0->p;
+  b->models.assign(models, models + nModels);
+  b->dt.assign(dt, dt + nModels);
+  auto fail = [&](const char *what) {
+    set_error("batch_create: %s", what);
+    selfamd_batch_destroy(b);
+    return (int)kHipError;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
+  for (int N = 1; N <= 7; N++) {
+    BatchGroup g;
+    g.N = N;
+    size_t total = 0;
+    for (int r = 0; r < nModels; r++)
+      if (models[r]->N == N) { g.runs.push_back(r); g.first.push_back(total); total += models[r]->nEl; }
+    if (g.runs.empty()) continue;
+    SELFAMD_REQUIRE(total < (size_t)1 << 27, "batch_create: degree-%d group has too many elements", N);
+    g.nEl = (int)total;
+    const size_t P = N + 1, PP = P * P, nV = m0->nVar, nT = m0->nT;
+    const size_t nS = total * PP * nV, nTr = total * P * nT * 4;
+    if (hipMalloc(&g.S, 8 * nS) || hipMalloc(&g.G, 8 * nS) || hipMalloc(&g.trace[0], 8 * nTr) ||
+        hipMalloc(&g.trace[1], 8 * nTr) || hipMalloc(&g.geom, 8 * kGeomStride * total) ||
+        hipMalloc(&g.mats, 8 * (2 * PP + 3 * P)) || hipMalloc(&g.nbr, 4 * 4 * total) ||
+        (m0->un && hipMalloc(&g.un, 8 * total * P * 8)) || (m0->aux && hipMalloc(&g.aux, 8 * total * PP)) ||
+        (m0->stat2 && hipMalloc(&g.stat2, 8 * total * PP)) || (m0->presTrace && hipMalloc(&g.presTrace, 8 * nTr))) {
+      b->groups.push_back(g);
+      return fail("device allocation failed");
+    }
+    b->groups.push_back(g);
+    BatchGroup &G = b->groups.back();
+    if (hipMemcpy(G.mats, models[G.runs[0]]->mats, 8 * (2 * PP + 3 * P), hipMemcpyDeviceToDevice))
+      return fail("copy failed");
+    std::vector<double> rec;
+    std::vector<int32_t> nb;
+    for (size_t k = 0; k < G.runs.size(); k++) {
+      const Model *m = models[G.runs[k]];
+      const size_t off = G.first[k], ne = m->nEl;
+      hipError_t e = hipSuccess;
+      e = e ? e : hipMemcpy(G.S + off * PP * nV, m->S, 8 * ne * PP * nV, hipMemcpyDeviceToDevice);
+      e = e ? e : hipMemcpy(G.G + off * PP * nV, m->G, 8 * ne * PP * nV, hipMemcpyDeviceToDevice);
+      e = e ? e : hipMemcpy(G.trace[0] + off * P * nT * 4, m->trace[m->cur], 8 * ne * P * nT * 4, hipMemcpyDeviceToDevice);
+      if (G.un) e = e ? e : hipMemcpy(G.un + off * P * 8, m->un, 8 * ne * P * 8, hipMemcpyDeviceToDevice);
+      if (G.aux) e = e ? e : hipMemcpy(G.aux + off * PP, m->aux, 8 * ne * PP, hipMemcpyDeviceToDevice);
+      if (G.stat2) e = e ? e : hipMemcpy(G.stat2 + off * PP, m->stat2, 8 * ne * PP, hipMemcpyDeviceToDevice);
+      if (G.presTrace)
+        e = e ? e : hipMemcpy(G.presTrace + off * P * nT * 4, m->presTrace, 8 * ne * P * nT * 4, hipMemcpyDeviceToDevice);
+      rec.resize(kGeomStride * ne);
+      nb.resize(4 * ne);
+      e = e ? e : hipMemcpy(rec.data(), m->geom, 8 * kGeomStride * ne, hipMemcpyDeviceToHost);
+      e = e ? e : hipMemcpy(nb.data(), m->nbr, 4 * 4 * ne, hipMemcpyDeviceToHost);
+      if (e) return fail("copy failed");
+      for (size_t q = 0; q < ne; q++) rec[kGeomStride * q + 5] = b->dt[G.runs[k]];
+      for (auto &c : nb)
+        if (c >= 0) c += (int32_t)(off << 3);  // neighbour element index shifts by the run's offset
+      e = hipMemcpy(G.geom + kGeomStride * off, rec.data(), 8 * kGeomStride * ne, hipMemcpyHostToDevice);
+      e = e ? e : hipMemcpy(G.nbr + 4 * off, nb.data(), 4 * 4 * ne, hipMemcpyHostToDevice);
+      if (e) return fail("copy failed");
+    }
+  }
+  // capture one RK3 step of every group, for both starting trace buffers
+  for (int start = 0; start < 2; start++) {
+    hipGraph_t gr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) return fail("capture failed");
+    hipError_t le = hipSuccess;
+    int tin = start;
+    for (int st = 0; st < 3; st++) {
+      for (auto &g : b->groups)
+        if (le == hipSuccess) le = b->launch_group(g, st, tin);
+      tin ^= 1;
+    }
+    hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+    if (le || ce) return fail("stage launch during capture failed");
+    hipError_t ie = hipGraphInstantiate(&b->graph[start], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie) return fail("graph instantiation failed");
+  }
+  *out = b;
+  return kOk;
+}
+
+int selfamd_batch_forward_step(selfamd_batch *b, int nSteps) {
+  SELFAMD_REQUIRE(b && nSteps >= 0, "batch_forward_step: bad arguments");
+  for (int n = 0; n < nSteps; n++) {
+    SELFAMD_HIP(hipGraphLaunch(b->graph[b->cur], b->stream));
+    b->cur ^= 1;
+  }
+  for (size_t r = 0; r < b->models.size(); r++) b->models[r]->t += nSteps * b->dt[r];
+  return kOk;
+}
+
+int selfamd_batch_time_steps(selfamd_batch *b, int nSteps, double *elapsedMs) {
+  SELFAMD_REQUIRE(b && elapsedMs, "batch_time_steps: null argument");
+  hipEvent_t e0, e1;
+  SELFAMD_HIP(hipEventCreate(&e0));
+  SELFAMD_HIP(hipEventCreate(&e1));
+  SELFAMD_HIP(hipEventRecord(e0, b->stream));
+  int rc = selfamd_batch_forward_step(b, nSteps);
+  if (rc) return rc;
+  SELFAMD_HIP(hipEventRecord(e1, b->stream));
+  SELFAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SELFAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *elapsedMs = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return kOk;
+}
+
+// Copies every run's state (solution, RK register, traces, lagged velocities) back into its
+// model handle, which then continues on its own exactly where the batch left it.
+int selfamd_batch_scatter(selfamd_batch *b) {
+  SELFAMD_REQUIRE(b, "batch_scatter: null batch");
+  SELFAMD_HIP(hipStreamSynchronize(b->stream));
+  for (auto &G : b->groups) {
+    const size_t P = G.N + 1, PP = P * P;
+    for (size_t k = 0; k < G.runs.size(); k++) {
+      Model *m = b->models[G.runs[k]];
+      const size_t off = G.first[k], ne = m->nEl, nV = m->nVar, nT = m->nT;
+      SELFAMD_HIP(hipMemcpy(m->S, G.S + off * PP * nV, 8 * ne * PP * nV, hipMemcpyDeviceToDevice));
+      SELFAMD_HIP(hipMemcpy(m->G, G.G + off * PP * nV, 8 * ne * PP * nV, hipMemcpyDeviceToDevice));
+      SELFAMD_HIP(hipMemcpy(m->trace[m->cur], G.trace[b->cur] + off * P * nT * 4, 8 * ne * P * nT * 4,
+                            hipMemcpyDeviceToDevice));
+      if (m->un) SELFAMD_HIP(hipMemcpy(m->un, G.un + off * P * 8, 8 * ne * P * 8, hipMemcpyDeviceToDevice));
+    }
+  }
+  return kOk;
+}
+
+int selfamd_batch_info(selfamd_batch *b, int *nGroups, double *totalDof) {
+  SELFAMD_REQUIRE(b && nGroups && totalDof, "batch_info: null argument");
+  *nGroups = (int)b->groups.size();
+  double dof = 0.0;
+  for (auto &g : b->groups) dof += (double)g.nEl * (g.N + 1) * (g.N + 1) * b->models[0]->nVar;
+  *totalDof = dof;
   return kOk;
 }
 

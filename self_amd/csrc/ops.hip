@@ -536,6 +536,13 @@ __global__ void k_euler_riemann(double *__restrict__ fbn, const double *__restri
   }
 }
 
+// Streaming copy dst = src (16-byte vector loads/stores): calibrates HBM counters and the
+// achievable bandwidth against kernels of known traffic.
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
 inline unsigned blocks_for(size_t n) {
   size_t b = (n + kBlock - 1) / kBlock;
   if (b > (1u << 20)) b = 1u << 20;  // grid-stride loops cover the rest
@@ -667,6 +674,17 @@ int selfamd_update_grk(double *G, double *Sv, const double *dSdt, double a, doub
   SELFAMD_CHECK_LAUNCH();
   return kOk;
 }
+int selfamd_copy_f64(const double *src, double *dst, size_t n, void *s) {
+  SELFAMD_REQUIRE(src && dst && n % 2 == 0, "copy_f64: null pointer or odd length %zu", n);
+  const size_t n2 = n / 2;
+  unsigned blocks = (unsigned)((n2 + kBlock - 1) / kBlock);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy, blocks ? blocks : 1, kBlock, 0, S(s), reinterpret_cast<const double2 *>(src),
+                     reinterpret_cast<double2 *>(dst), n2);
+  SELFAMD_CHECK_LAUNCH();
+  return kOk;
+}
+
 int selfamd_calculate_dsdt(const double *fd, const double *src, double *d, int N, int nVar, int nEl, void *s) {
   CHECK_DIMS(N, nVar, nEl);
   const size_t n = (size_t)(N + 1) * (N + 1) * nVar * nEl;

@@ -71,7 +71,7 @@ CONTAINS
       INTEGER(C_SIZE_T) :: nb
       nb = INT(SIZE(a), C_SIZE_T)*8_C_SIZE_T
       p = device_alloc(nb)
-      CALL hip_check(hipMemcpy(p, C_LOC(a), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+      CALL hip_check(selfamd_memcpy(p, C_LOC(a), nb, hipMemcpyHostToDevice), 'selfamd_memcpy')
     END SUBROUTINE to_device
     SUBROUTINE to_device1(a, p)
       REAL(prec), INTENT(in), TARGET :: a(:)
@@ -79,7 +79,7 @@ CONTAINS
       INTEGER(C_SIZE_T) :: nb
       nb = INT(SIZE(a), C_SIZE_T)*8_C_SIZE_T
       p = device_alloc(nb)
-      CALL hip_check(hipMemcpy(p, C_LOC(a), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+      CALL hip_check(selfamd_memcpy(p, C_LOC(a), nb, hipMemcpyHostToDevice), 'selfamd_memcpy')
     END SUBROUTINE to_device1
   END SUBROUTINE Init_Lagrange
 

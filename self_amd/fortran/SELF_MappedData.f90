@@ -69,16 +69,16 @@ CONTAINS
 
   SUBROUTINE UpdateHost_MappedScalar2D(this)
     CLASS(MappedScalar2D), INTENT(inout), TARGET :: this
-    CALL hip_check(hipMemcpy(C_LOC(this % interior), this % interior_dev, &
-                   INT(SIZE(this % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
-    CALL hip_check(hipMemcpy(C_LOC(this % boundary), this % boundary_dev, &
-                   INT(SIZE(this % boundary), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(this % interior), this % interior_dev, &
+                   INT(SIZE(this % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'selfamd_memcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(this % boundary), this % boundary_dev, &
+                   INT(SIZE(this % boundary), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'selfamd_memcpy')
   END SUBROUTINE UpdateHost_MappedScalar2D
 
   SUBROUTINE UpdateDevice_MappedScalar2D(this)
     CLASS(MappedScalar2D), INTENT(inout), TARGET :: this
-    CALL hip_check(hipMemcpy(this % interior_dev, C_LOC(this % interior), &
-                   INT(SIZE(this % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyHostToDevice), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(this % interior_dev, C_LOC(this % interior), &
+                   INT(SIZE(this % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyHostToDevice), 'selfamd_memcpy')
   END SUBROUTINE UpdateDevice_MappedScalar2D
 
   ! BoundaryInterp_Scalar2D (src/SELF_Data.f90:630-647), device branch

@@ -83,7 +83,7 @@ CONTAINS
     INTEGER(C_SIZE_T) :: nb
     nb = INT(SIZE(myMesh % sideInfo), C_SIZE_T)*4_C_SIZE_T
     IF (.NOT. C_ASSOCIATED(myMesh % sideInfo_dev)) myMesh % sideInfo_dev = device_alloc(nb)
-    CALL hip_check(hipMemcpy(myMesh % sideInfo_dev, C_LOC(myMesh % sideInfo), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(myMesh % sideInfo_dev, C_LOC(myMesh % sideInfo), nb, hipMemcpyHostToDevice), 'selfamd_memcpy')
   END SUBROUTINE UpdateDevice_Mesh2D
 
   SUBROUTINE Free_Mesh2D(myMesh)
@@ -122,7 +122,7 @@ CONTAINS
     INTEGER(C_SIZE_T) :: nb
     nb = INT(SIZE(mesh % nodeCoords), C_SIZE_T)*8_C_SIZE_T
     nc = device_alloc(nb)
-    CALL hip_check(hipMemcpy(nc, C_LOC(mesh % nodeCoords), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(nc, C_LOC(mesh % nodeCoords), nb, hipMemcpyHostToDevice), 'selfamd_memcpy')
     CALL selfamd_check(selfamd_semquad_generate(INT(myGeom % interp % N, C_INT), &
                        INT(myGeom % interp % controlNodeType, C_INT), INT(mesh % nGeo, C_INT), &
                        INT(mesh % nElem, C_INT), nc, myGeom % x, myGeom % xb, myGeom % dxds, myGeom % dxdsb, &
@@ -135,8 +135,8 @@ CONTAINS
   SUBROUTINE CopyX_SEMQuad(myGeom, x)
     CLASS(SEMQuad), INTENT(in) :: myGeom
     REAL(prec), INTENT(out), TARGET :: x(:,:,:,:)
-    CALL hip_check(hipMemcpy(C_LOC(x), myGeom % x, INT(SIZE(x), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
-                   'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(x), myGeom % x, INT(SIZE(x), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
+                   'selfamd_memcpy')
   END SUBROUTINE CopyX_SEMQuad
 
   ! CovariantArcMin_SEMQuad (src/SELF_Geometry.f90:554-582)
@@ -148,8 +148,8 @@ CONTAINS
     INTEGER :: i, j, e, N
     N = myGeom % interp % N
     ALLOCATE (dxds(2, 2, 0:N, 0:N, myGeom % nElem))
-    CALL hip_check(hipMemcpy(C_LOC(dxds), myGeom % dxds, INT(SIZE(dxds), C_SIZE_T)*8_C_SIZE_T, &
-                             hipMemcpyDeviceToHost), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(dxds), myGeom % dxds, INT(SIZE(dxds), C_SIZE_T)*8_C_SIZE_T, &
+                             hipMemcpyDeviceToHost), 'selfamd_memcpy')
     dxMin = HUGE(1.0_prec)
     DO e = 1, myGeom % nElem
       DO j = 0, N

@@ -16,6 +16,7 @@ MODULE SELF_Model2D
 
   INTEGER, PARAMETER :: SELF_MODEL_LSW = 1, SELF_MODEL_SW = 2, SELF_MODEL_EULER = 3
   INTEGER(C_INT), PARAMETER :: PARAM_G = 1, PARAM_FCORI = 2, PARAM_GAMMA = 3, PARAM_RGAS = 4
+  INTEGER(C_INT), PARAMETER :: PARAM_NODAL_METRICS = 5
   INTEGER(C_INT), PARAMETER :: FIELD_TOPOGRAPHY = 1, FIELD_BATHYMETRY = 2, FIELD_CORIOLIS = 3
   INTEGER(C_INT), PARAMETER :: FIELD_POTENTIAL = 4, FIELD_DRAG = 5
 
@@ -36,6 +37,7 @@ MODULE SELF_Model2D
     PROCEDURE, PUBLIC :: CalculateTendency => CalculateTendency_Model2D
     PROCEDURE, PUBLIC :: ForwardStep => ForwardStep_Model2D
     PROCEDURE, PUBLIC :: UpdateHost => UpdateHost_Model2D
+    PROCEDURE, PUBLIC :: SetNodalMetrics => SetNodalMetrics_Model2D
     PROCEDURE, PRIVATE :: SetField => SetField_Model2D
   END TYPE Model2D
 
@@ -90,6 +92,15 @@ CONTAINS
     CALL this % solution % Free()
     CALL this % dSdt % Free()
   END SUBROUTINE Free_Model2D
+
+  ! Per-node metric terms (the reference's representation, needed on curved meshes) or one
+  ! affine record per element (the default on affine meshes)
+  SUBROUTINE SetNodalMetrics_Model2D(this, nodal)
+    CLASS(Model2D), INTENT(inout) :: this
+    LOGICAL, INTENT(in) :: nodal
+    CALL selfamd_check(selfamd_model2d_set_param(this % engine, PARAM_NODAL_METRICS, &
+                       MERGE(1.0_prec, 0.0_prec, nodal)), 'Model2D%SetNodalMetrics')
+  END SUBROUTINE SetNodalMetrics_Model2D
 
   ! SetSolution (src/SELF_Model2D.f90:313-332) from an array interior(0:N,0:N,nVar,nElem)
   SUBROUTINE SetSolution_Model2D(this, interior)
@@ -155,10 +166,10 @@ CONTAINS
     CLASS(Model2D), INTENT(inout), TARGET :: this
     CALL selfamd_check(selfamd_model2d_get_solution(this % engine, this % solution % interior_dev), &
                        'Model2D%UpdateHost')
-    CALL hip_check(hipMemcpy(C_LOC(this % solution % interior), this % solution % interior_dev, &
-                   INT(SIZE(this % solution % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
-    CALL hip_check(hipMemcpy(C_LOC(this % dSdt % interior), this % dSdt % interior_dev, &
-                   INT(SIZE(this % dSdt % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(this % solution % interior), this % solution % interior_dev, &
+                   INT(SIZE(this % solution % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'selfamd_memcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(this % dSdt % interior), this % dSdt % interior_dev, &
+                   INT(SIZE(this % dSdt % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'selfamd_memcpy')
   END SUBROUTINE UpdateHost_Model2D
 
   SUBROUTINE SetField_Model2D(this, which, values)
@@ -169,7 +180,7 @@ CONTAINS
     INTEGER(C_SIZE_T) :: nb
     nb = INT(SIZE(values), C_SIZE_T)*8_C_SIZE_T
     d = device_alloc(nb)
-    CALL hip_check(hipMemcpy(d, C_LOC(values), nb, hipMemcpyHostToDevice), 'hipMemcpy')
+    CALL hip_check(selfamd_memcpy(d, C_LOC(values), nb, hipMemcpyHostToDevice), 'selfamd_memcpy')
     CALL selfamd_check(selfamd_model2d_set_field(this % engine, which, d), 'Model2D%SetField')
     CALL device_free(d)
   END SUBROUTINE SetField_Model2D

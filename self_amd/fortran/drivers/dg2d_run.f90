@@ -1,0 +1,108 @@
+! dg2d_run -- runs one 2D DGSEM model on a structured mesh through the Fortran API, in the
+! manner of the reference's example programs (test/ShallowWater2D_*.f90): Lagrange%Init,
+! Mesh2D, SEMQuad%GenerateFromMesh, <Model>%Init, Set<Field>, SetSolution,
+! CalculateTendency, ForwardStep.
+!
+! Usage: dg2d_run <parameter file>.  The namelist &run gives the model (1 LSW, 2 SW,
+! 3 Euler), N, nx, ny, periodic, bcid, nSteps, dt, nodal (metric representation) and
+! prefix; <prefix>_sol.bin holds the initial interior(0:N,0:N,nVar,nElem) and
+! <prefix>_field1.bin / _field2.bin the static fields (0:N,0:N,nElem) in native doubles.
+! Outputs: <prefix>_tend.bin (tendency of the initial state) and <prefix>_out.bin
+! (solution after nSteps RK3 steps).
+PROGRAM dg2d_run
+  USE SELF_Constants
+  USE SELF_Lagrange
+  USE SELF_Mesh
+  USE SELF_Model2D
+  IMPLICIT NONE
+  INTEGER :: model = 2, N = 7, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
+  LOGICAL :: periodic = .FALSE., nodal = .FALSE.
+  REAL(prec) :: dt = 1.0e-4_prec
+  CHARACTER(LEN=400) :: prefix = 'dg2d', parfile
+  NAMELIST /run/ model, N, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix
+  TYPE(Lagrange), TARGET :: interp
+  TYPE(Mesh2D), TARGET :: mesh
+  TYPE(SEMQuad), TARGET :: geometry
+  TYPE(ShallowWater) :: sw
+  TYPE(LinearShallowWater) :: lsw
+  TYPE(CompressibleIdealGas2D) :: euler
+  REAL(prec), ALLOCATABLE :: sol(:,:,:,:), f1(:,:,:), f2(:,:,:)
+  INTEGER :: u, nVar
+
+  CALL GET_COMMAND_ARGUMENT(1, parfile)
+  OPEN (NEWUNIT=u, FILE=TRIM(parfile), STATUS='old', ACTION='read')
+  READ (u, NML=run)
+  CLOSE (u)
+  nVar = MERGE(4, 3, model == 3)
+
+  CALL interp % Init(N, GAUSS, N, GAUSS)
+  CALL mesh % StructuredMesh(nx, ny, 0.0_prec, 1.0_prec, 0.0_prec, 1.0_prec, periodic, periodic, bcid)
+  CALL geometry % Init(interp, mesh % nElem)
+  CALL geometry % GenerateFromMesh(mesh)
+
+  ALLOCATE (sol(0:N, 0:N, nVar, mesh % nElem), f1(0:N, 0:N, mesh % nElem), f2(0:N, 0:N, mesh % nElem))
+  CALL read_array(TRIM(prefix)//'_sol.bin', sol, SIZE(sol))
+  CALL read_array(TRIM(prefix)//'_field1.bin', f1, SIZE(f1))
+  CALL read_array(TRIM(prefix)//'_field2.bin', f2, SIZE(f2))
+
+  SELECT CASE (model)
+  CASE (1)
+    CALL lsw % Init(nVar, mesh, geometry)
+    CALL lsw % SetNodalMetrics(nodal)
+    CALL lsw % SetBathymetry(f1)
+    CALL lsw % SetCoriolis(f2)
+    CALL run_model(lsw)
+  CASE (2)
+    CALL sw % Init(nVar, mesh, geometry)
+    CALL sw % SetNodalMetrics(nodal)
+    CALL sw % SetTopography(f1)
+    CALL run_model(sw)
+  CASE (3)
+    CALL euler % Init(nVar, mesh, geometry)
+    CALL euler % SetNodalMetrics(nodal)
+    CALL euler % SetGravity(f1)
+    CALL euler % SetDrag(f2)
+    CALL run_model(euler)
+  CASE DEFAULT
+    PRINT *, 'dg2d_run: unknown model ', model
+    ERROR STOP 2
+  END SELECT
+  PRINT '(A,I0,A,I0,A)', 'dg2d_run: model ', model, ' done (', nSteps, ' RK3 steps)'
+
+CONTAINS
+
+  SUBROUTINE run_model(m)
+    CLASS(Model2D), INTENT(inout) :: m
+    CALL m % SetTimeIntegrator('rk3')
+    CALL m % SetSolution(sol)
+    CALL m % CalculateTendency()
+    CALL m % UpdateHost()
+    CALL write_array(TRIM(prefix)//'_tend.bin', m % dSdt % interior, SIZE(m % dSdt % interior))
+    m % t = 0.0_prec
+    CALL m % ForwardStep(tn=REAL(nSteps, prec)*dt, dt=dt)
+    CALL m % UpdateHost()
+    CALL write_array(TRIM(prefix)//'_out.bin', m % solution % interior, SIZE(m % solution % interior))
+    CALL m % Free()
+  END SUBROUTINE run_model
+
+  SUBROUTINE read_array(path, a, n)
+    CHARACTER(*), INTENT(in) :: path
+    INTEGER, INTENT(in) :: n
+    REAL(prec), INTENT(out) :: a(n)
+    INTEGER :: v
+    OPEN (NEWUNIT=v, FILE=path, ACCESS='stream', FORM='unformatted', STATUS='old', ACTION='read')
+    READ (v) a
+    CLOSE (v)
+  END SUBROUTINE read_array
+
+  SUBROUTINE write_array(path, a, n)
+    CHARACTER(*), INTENT(in) :: path
+    INTEGER, INTENT(in) :: n
+    REAL(prec), INTENT(in) :: a(n)
+    INTEGER :: v
+    OPEN (NEWUNIT=v, FILE=path, ACCESS='stream', FORM='unformatted', STATUS='replace', ACTION='write')
+    WRITE (v) a
+    CLOSE (v)
+  END SUBROUTINE write_array
+
+END PROGRAM dg2d_run

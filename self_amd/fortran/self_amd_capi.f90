@@ -1,5 +1,5 @@
-! self_amd_capi.f90 -- iso_c_binding interfaces to libself_amd.so (include/self_amd.h) and the
-! few HIP runtime calls the Fortran host layer needs (device allocation and copies).
+! self_amd_capi.f90 -- iso_c_binding interfaces to libself_amd.so (include/self_amd.h),
+! including its device allocation and copy entry points.
 !
 ! Device pointers are passed BY VALUE (type(c_ptr), value); every entry point returns an
 ! integer status (0 = success).  selfamd_check stops with the library's message otherwise.
@@ -7,34 +7,35 @@ MODULE SELF_AMD_CAPI
   USE ISO_C_BINDING
   IMPLICIT NONE
 
+  ! selfamd_memcpy kinds
   INTEGER(C_INT), PARAMETER :: hipMemcpyHostToDevice = 1
   INTEGER(C_INT), PARAMETER :: hipMemcpyDeviceToHost = 2
   INTEGER(C_INT), PARAMETER :: hipMemcpyDeviceToDevice = 3
 
   INTERFACE
-    ! ---- HIP runtime (libamdhip64) ----
-    INTEGER(C_INT) FUNCTION hipMalloc(ptr, bytes) BIND(C, name="hipMalloc")
+    ! ---- device plumbing (libself_amd; the process loads only the HIP runtime it links) ----
+    INTEGER(C_INT) FUNCTION selfamd_malloc(ptr, bytes) BIND(C, name="selfamd_malloc")
       IMPORT :: C_INT, C_PTR, C_SIZE_T
       TYPE(C_PTR) :: ptr
       INTEGER(C_SIZE_T), VALUE :: bytes
     END FUNCTION
-    INTEGER(C_INT) FUNCTION hipFree(ptr) BIND(C, name="hipFree")
+    INTEGER(C_INT) FUNCTION selfamd_free(ptr) BIND(C, name="selfamd_free")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ptr
     END FUNCTION
-    INTEGER(C_INT) FUNCTION hipMemcpy(dst, src, bytes, kind) BIND(C, name="hipMemcpy")
+    INTEGER(C_INT) FUNCTION selfamd_memcpy(dst, src, bytes, kind) BIND(C, name="selfamd_memcpy")
       IMPORT :: C_INT, C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: dst, src
       INTEGER(C_SIZE_T), VALUE :: bytes
       INTEGER(C_INT), VALUE :: kind
     END FUNCTION
-    INTEGER(C_INT) FUNCTION hipMemset(ptr, val, bytes) BIND(C, name="hipMemset")
+    INTEGER(C_INT) FUNCTION selfamd_memset(ptr, val, bytes) BIND(C, name="selfamd_memset")
       IMPORT :: C_INT, C_PTR, C_SIZE_T
       TYPE(C_PTR), VALUE :: ptr
       INTEGER(C_INT), VALUE :: val
       INTEGER(C_SIZE_T), VALUE :: bytes
     END FUNCTION
-    INTEGER(C_INT) FUNCTION hipDeviceSynchronize() BIND(C, name="hipDeviceSynchronize")
+    INTEGER(C_INT) FUNCTION selfamd_device_synchronize() BIND(C, name="selfamd_device_synchronize")
       IMPORT :: C_INT
     END FUNCTION
 
@@ -210,26 +211,24 @@ CONTAINS
   END SUBROUTINE selfamd_check
 
   SUBROUTINE hip_check(rc, what)
+    !! Device plumbing calls report through the same channel as the other entry points
     INTEGER(C_INT), INTENT(in) :: rc
     CHARACTER(*), INTENT(in) :: what
-    IF (rc /= 0) THEN
-      PRINT '(A,A,A,I0)', 'self_amd: HIP call ', what, ' failed with hipError ', rc
-      ERROR STOP 3
-    END IF
+    CALL selfamd_check(rc, what)
   END SUBROUTINE hip_check
 
   FUNCTION device_alloc(nbytes) RESULT(p)
-    !! hipMalloc + zero fill
+    !! Device allocation, zero filled
     INTEGER(C_SIZE_T), INTENT(in) :: nbytes
     TYPE(C_PTR) :: p
     p = C_NULL_PTR
-    CALL hip_check(hipMalloc(p, MAX(nbytes, 8_C_SIZE_T)), 'hipMalloc')
-    CALL hip_check(hipMemset(p, 0_C_INT, MAX(nbytes, 8_C_SIZE_T)), 'hipMemset')
+    CALL selfamd_check(selfamd_malloc(p, MAX(nbytes, 8_C_SIZE_T)), 'selfamd_malloc')
+    CALL selfamd_check(selfamd_memset(p, 0_C_INT, MAX(nbytes, 8_C_SIZE_T)), 'selfamd_memset')
   END FUNCTION device_alloc
 
   SUBROUTINE device_free(p)
     TYPE(C_PTR), INTENT(inout) :: p
-    IF (C_ASSOCIATED(p)) CALL hip_check(hipFree(p), 'hipFree')
+    IF (C_ASSOCIATED(p)) CALL selfamd_check(selfamd_free(p), 'selfamd_free')
     p = C_NULL_PTR
   END SUBROUTINE device_free
 
