@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=100, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for HBM counter runs: 3 dispatches of a 1 GiB FP64 copy (calibration), "
+                         "then 2 RK3 steps (6 stage launches) of the workload; prints no JSON")
     return ap.parse_args()
 
 
@@ -47,6 +50,26 @@ def traffic_from_profiles(root):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
+
+
+def traffic_probe(self_amd, model, dt):
+    """Known-traffic copy kernel (1 GiB read + 1 GiB written per dispatch), then the stage
+    kernel, so FETCH_SIZE / WRITE_SIZE of the stage launches can be scaled by what the same
+    counters read for the copy."""
+    from self_amd._lib import call
+    n = 1 << 27
+    print(f"probe: copy {8 * n} bytes x3", flush=True)
+    a = self_amd.DeviceArray((n,), zero=True)
+    b = self_amd.DeviceArray((n,), zero=True)
+    for _ in range(3):
+        call("selfamd_copy_f64", a.ptr, b.ptr, n, None)
+    self_amd.synchronize()
+    del a, b
+    print("probe: 2 RK3 steps", flush=True)
+    model.ForwardStep(2, dt)
+    self_amd.synchronize()
+    print("probe: done", flush=True)
+    return 0
 
 
 def main():
@@ -72,6 +95,9 @@ def main():
     speed = float((np.sqrt(sol[:, 0] ** 2 + sol[:, 1] ** 2) / H + np.sqrt(H)).max())
     dt = workloads.cfl_dt(geom.CovariantArcMin(), speed, 0.5)
     dof = mesh.nElem * (N + 1) ** 2 * 3
+
+    if a.traffic_probe:
+        return traffic_probe(self_amd, model, dt)
 
     model.ForwardStep(a.warmup, dt)
     self_amd.synchronize()
@@ -114,7 +140,7 @@ def main():
                    "dof": dof, "dt": dt, "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(root),
-                     "kernel": "k_fused<PhysSW,7,0>", "avg_launch_us": launch_s * 1e6,
+                     "kernel": "k_fused<PhysSW, 7, 0, false, 64>", "avg_launch_us": launch_s * 1e6,
                      "bytes_per_launch_alg": BYTES_PER_DOF_UPDATE / 3.0 * dof},
         "cpu_baseline": cpu,
     }

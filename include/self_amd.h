@@ -290,6 +290,15 @@ int selfamd_model2d_synchronize(selfamd_model2d *model);
 /* forward_step bracketed by two hipEvents on the model's stream; returns the elapsed
  * milliseconds between them (synchronises). */
 int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, double *elapsedMs);
+/* Tile variants of the nonlinear-SW stage kernel (affine metrics) at N = 3, 5, 7: block size,
+ * occupancy target and XCD-contiguous element order.  No reference counterpart; the
+ * autotuner (self_amd/autotune.py) times them and records the fastest per N.  A set variant
+ * (-1 = default tile) applies to every model's next graph capture and to batches created
+ * afterwards; it never changes results (same arithmetic per element). */
+int selfamd_tile_count(int kind, int N, int *count);
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *wavesPerEU, int *xcdOrder);
+int selfamd_tile_set(int kind, int N, int variant);
+int selfamd_tile_get(int kind, int N, int *variant);
 /* Model time t */
 int selfamd_model2d_get_time(selfamd_model2d *model, double *t);
 int selfamd_model2d_set_time(selfamd_model2d *model, double t);
@@ -297,7 +306,7 @@ int selfamd_model2d_set_time(selfamd_model2d *model, double t);
 int selfamd_model2d_nvar(int kind);
 
 /* ------------------------------------------------------------------------------------ */
-/* Batched runs (configuration C4): many independent models advanced together            */
+/* Batched runs: many independent models advanced together                               */
 /* ------------------------------------------------------------------------------------ */
 /* No reference counterpart (the reference runs one model per process).  Models of the same
  * kind, affine metrics and identical physical parameters; runs of one polynomial degree are
@@ -312,6 +321,48 @@ int selfamd_batch_time_steps(selfamd_batch *batch, int nSteps, double *elapsedMs
 int selfamd_batch_scatter(selfamd_batch *batch);
 int selfamd_batch_info(selfamd_batch *batch, int *nGroups, double *totalDof);
 int selfamd_batch_destroy(selfamd_batch *batch);
+
+/* ------------------------------------------------------------------------------------ */
+/* Learned subgrid closure: FP32 MLP, fine -> coarse projection, model hook               */
+/* ------------------------------------------------------------------------------------ */
+/* No reference counterpart.  An MLP n_in -> H -> H -> n_out with GELU hidden layers, FP32
+ * weights and FP32 accumulation, trained by the library's own forward/backward kernels
+ * (mean-squared-error loss, Adam).  Parameters are packed on the host as
+ * W1[n_in][H] b1[H] W2[H][H] b2[H] W3[H][n_out] b3[n_out] (row-major W[in][out], so a layer
+ * is Y = X W + b).  Rows of X and Y are samples, device memory, row-major. */
+typedef struct selfamd_mlp selfamd_mlp;
+int selfamd_mlp_create(selfamd_mlp **mlp, int nIn, int nHidden, int nOut, int maxBatch);
+int selfamd_mlp_destroy(selfamd_mlp *mlp);
+int selfamd_mlp_num_params(selfamd_mlp *mlp, size_t *nParams);
+/* Loads parameters (host) and resets the Adam moments. */
+int selfamd_mlp_set_params(selfamd_mlp *mlp, const float *params);
+int selfamd_mlp_get_params(selfamd_mlp *mlp, float *params);
+/* Gradients of the last selfamd_mlp_loss_grad call, packed like the parameters (host). */
+int selfamd_mlp_get_grads(selfamd_mlp *mlp, float *grads);
+int selfamd_mlp_forward(selfamd_mlp *mlp, const float *X_dev, float *Y_dev, int batch, void *stream);
+/* Forward + backward of L = mean((Y - T)^2) over batch rows; *loss (host, may be null) is the
+ * loss of this batch (synchronises the stream when requested). */
+int selfamd_mlp_loss_grad(selfamd_mlp *mlp, const float *X_dev, const float *T_dev, int batch, float *loss,
+                          void *stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected; step counts from 1). */
+int selfamd_mlp_adam_step(selfamd_mlp *mlp, float lr, int step, void *stream);
+
+/* L2 projection of interior(0:Nf,0:Nf,nVar,(r nxc)(r nyc)) on a structured raster-ordered
+ * mesh onto interior(0:Nc,0:Nc,nVar,nxc nyc): each coarse element covers r x r fine elements.
+ * Exact for coarse-space polynomials and mass-conserving. */
+int selfamd_project_l2_structured_2d(const double *fine_dev, double *coarse_dev, int Nf, int Nc, int nxc, int nyc,
+                                     int r, int nVar, void *stream);
+
+/* Attaches an MLP as a tendency correction: before each stage (and in CalculateTendency) the
+ * element features x = ((S nodal values, J, dsdx(1,1), dsdx(2,2)) - inMean) / inScale go
+ * through the network, and dS/dt += y * outScale + outMean, with the mass variable's
+ * correction made mean-free over the mesh (mass is conserved to round-off).  Needs an affine
+ * mesh and n_in = nVar (N+1)^2 + 3, n_out = nVar (N+1)^2, maxBatch >= nEl.  Normalisation
+ * arrays are host memory; mlp = NULL detaches.  The model keeps a pointer to the network. */
+int selfamd_model2d_set_closure(selfamd_model2d *model, selfamd_mlp *mlp, const float *inMean, const float *inScale,
+                                const float *outMean, const float *outScale);
+/* The closure's normalised features of the current state: X(nEl, n_in) FP32 device. */
+int selfamd_model2d_closure_features(selfamd_model2d *model, float *X_dev);
 
 #ifdef __cplusplus
 }

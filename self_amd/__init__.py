@@ -302,6 +302,30 @@ class Model2D:
         call("selfamd_model2d_time_steps", self._h, int(nSteps), float(dt), ctypes.byref(ms))
         return ms.value
 
+    def solution_device(self):
+        """Device copy of the solution (a buffer owned by the model, overwritten by the next
+        solution/tendency call)."""
+        call("selfamd_model2d_get_solution", self._h, _ptr(self._buf))
+        return self._buf
+
+    def SetClosure(self, mlp, inMean=None, inScale=None, outMean=None, outScale=None):
+        """Attach an MLP tendency correction (None detaches); normalisation arrays are host
+        float32 of the network's input/output widths."""
+        if mlp is None:
+            call("selfamd_model2d_set_closure", self._h, None, None, None, None, None)
+            self._closure = None
+            return
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (inMean, inScale, outMean, outScale)]
+        call("selfamd_model2d_set_closure", self._h, mlp._h, *[_ptr(a) for a in arrs])
+        self._closure = (mlp, arrs)
+
+    def closure_features(self):
+        """Host copy of the attached closure's normalised features of the current state."""
+        mlp = self._closure[0]
+        X = DeviceArray((self.nElem, mlp.nIn), np.float32)
+        call("selfamd_model2d_closure_features", self._h, _ptr(X))
+        return X.to_host()
+
     @property
     def t(self):
         v = ctypes.c_double()
@@ -366,7 +390,7 @@ class CompressibleIdealGas2D(Model2D):
 
 
 class Batch:
-    """Many independent models advanced together (no reference counterpart; configuration C4).
+    """Many independent models advanced together (no reference counterpart).
 
     Runs of one polynomial degree share one fused launch per RK stage and every degree group
     of a step sits in one hipGraph.  `scatter()` hands each run's state back to its model."""
@@ -403,3 +427,94 @@ class Batch:
             except Exception:
                 pass
             self._h = None
+
+
+# ----------------------------------------------------------------------------------------
+# Learned subgrid closure (no reference counterpart)
+# ----------------------------------------------------------------------------------------
+MLP_KEYS = ("W1", "b1", "W2", "b2", "W3", "b3")
+
+
+class MLP:
+    """FP32 MLP nIn -> nHidden -> nHidden -> nOut (GELU), trained by the library's kernels.
+
+    Parameters are exchanged as a dict of float32 arrays W1[nIn, H], b1[H], W2[H, H], b2[H],
+    W3[H, nOut], b3[nOut] (a layer is Y = X W + b)."""
+
+    def __init__(self, nIn, nHidden, nOut, maxBatch):
+        self.nIn, self.nHidden, self.nOut, self.maxBatch = nIn, nHidden, nOut, maxBatch
+        _require_device()
+        self._h = ctypes.c_void_p()
+        call("selfamd_mlp_create", ctypes.byref(self._h), nIn, nHidden, nOut, maxBatch)
+        n = ctypes.c_size_t()
+        call("selfamd_mlp_num_params", self._h, ctypes.byref(n))
+        self.nParams = n.value
+
+    def _shapes(self):
+        i, h, o = self.nIn, self.nHidden, self.nOut
+        return {"W1": (i, h), "b1": (h,), "W2": (h, h), "b2": (h,), "W3": (h, o), "b3": (o,)}
+
+    def pack(self, params):
+        return np.concatenate([np.asarray(params[k], np.float32).ravel() for k in MLP_KEYS])
+
+    def unpack(self, flat):
+        out, o = {}, 0
+        for k, shp in self._shapes().items():
+            n = int(np.prod(shp))
+            out[k] = flat[o:o + n].reshape(shp)
+            o += n
+        return out
+
+    def set_params(self, params):
+        """Load parameters (dict) and reset the optimiser moments."""
+        flat = self.pack(params)
+        assert flat.size == self.nParams
+        call("selfamd_mlp_set_params", self._h, _ptr(flat))
+
+    def get_params(self):
+        flat = np.empty(self.nParams, np.float32)
+        call("selfamd_mlp_get_params", self._h, _ptr(flat))
+        return self.unpack(flat)
+
+    def get_grads(self):
+        flat = np.empty(self.nParams, np.float32)
+        call("selfamd_mlp_get_grads", self._h, _ptr(flat))
+        return self.unpack(flat)
+
+    def forward(self, X):
+        """Y = net(X) for X (rows, nIn) host or DeviceArray; returns a host array."""
+        Xd = X if isinstance(X, DeviceArray) else DeviceArray.from_host(np.asarray(X, np.float32))
+        rows = Xd.shape[0]
+        Y = DeviceArray((rows, self.nOut), np.float32)
+        call("selfamd_mlp_forward", self._h, Xd.ptr, Y.ptr, rows, None)
+        return Y.to_host()
+
+    def loss_grad(self, X, T, rows=None, x_offset=0, want_loss=True):
+        """Forward + backward on rows [x_offset, x_offset + rows) of device arrays X, T."""
+        rows = X.shape[0] if rows is None else rows
+        xp = ctypes.c_void_p(X.ptr.value + 4 * self.nIn * x_offset)
+        tp = ctypes.c_void_p(T.ptr.value + 4 * self.nOut * x_offset)
+        loss = ctypes.c_float()
+        call("selfamd_mlp_loss_grad", self._h, xp, tp, int(rows), ctypes.byref(loss) if want_loss else None, None)
+        return loss.value if want_loss else None
+
+    def adam_step(self, lr, step):
+        call("selfamd_mlp_adam_step", self._h, float(lr), int(step), None)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load().selfamd_mlp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def project_l2_structured(fine, Nf, Nc, nxc, nyc, r, nVar):
+    """L2 projection of a structured fine-mesh interior (DeviceArray or numpy, shape
+    (nElem_f, nVar, Nf+1, Nf+1)) onto the coarse mesh; returns a DeviceArray."""
+    f = _dev(fine)
+    out = DeviceArray((nxc * nyc, nVar, Nc + 1, Nc + 1))
+    call("selfamd_project_l2_structured_2d", f.ptr, out.ptr, Nf, Nc, nxc, nyc, r, nVar, None)
+    return out

@@ -88,6 +88,22 @@ SIGNATURES = {
     "selfamd_batch_scatter": [c_void_p],
     "selfamd_batch_info": [c_void_p, c_void_p, c_void_p],
     "selfamd_batch_destroy": [c_void_p],
+    "selfamd_mlp_create": [c_void_p, c_int, c_int, c_int, c_int],
+    "selfamd_mlp_destroy": [c_void_p],
+    "selfamd_mlp_num_params": [c_void_p, c_void_p],
+    "selfamd_mlp_set_params": [c_void_p, c_void_p],
+    "selfamd_mlp_get_params": [c_void_p, c_void_p],
+    "selfamd_mlp_get_grads": [c_void_p, c_void_p],
+    "selfamd_mlp_forward": [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
+    "selfamd_mlp_loss_grad": [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
+    "selfamd_mlp_adam_step": [c_void_p, ctypes.c_float, c_int, c_void_p],
+    "selfamd_project_l2_structured_2d": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    "selfamd_model2d_set_closure": [c_void_p] * 6,
+    "selfamd_model2d_closure_features": [c_void_p, c_void_p],
+    "selfamd_tile_count": [c_int, c_int, c_void_p],
+    "selfamd_tile_info": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "selfamd_tile_set": [c_int, c_int, c_int],
+    "selfamd_tile_get": [c_int, c_int, c_void_p],
 }
 
 

@@ -1,0 +1,220 @@
+"""Learned subgrid closure for nonlinear shallow water 2D (no reference counterpart).
+
+A coarse run (nxc x nxc periodic elements, degree Nc) is corrected by an FP32 MLP that
+maps each element's nodal state plus its metric scalars (J, dsdx(1,1), dsdx(2,2)) to a
+tendency correction.  Training data come from fine runs of the same solver (r nxc x r nxc
+elements, degree Nf, the seeded nonlinear-SW distribution): for a sampled coarse step
+starting at the projected fine state P S_f(t),
+
+    target = (P S_f(t + dt_c) - Phi_c(P S_f(t))) / dt_c,
+
+where Phi_c is one plain coarse RK3 step and dt_c = k dt_f (k whole fine steps).  Because an
+RK3 step adds exactly dt_c times a constant forcing, a correction equal to the target would
+make the coarse step land on the projected fine state.  The library's kernels do all of it:
+the fine and coarse solvers, the L2 projection, the MLP forward/backward and Adam; this
+module drives them.  The mass (H) correction is made mean-free over the mesh by the engine,
+so the corrected run conserves mass to round-off.
+"""
+import numpy as np
+
+from . import (GAUSS, DeviceArray, Lagrange, MLP, Mesh2D, SEMQuad, ShallowWater, project_l2_structured, workloads)
+
+NVAR = 3
+
+
+class CoarseFine:
+    """The fine and coarse discretisations of the unit periodic square and their time steps.
+
+    dt_f = cfl * arcmin_f / speed and dt_c = k dt_f with k = floor(cfl * arcmin_c / speed / dt_f),
+    with one wave-speed bound for all runs so that every run shares the step sizes."""
+
+    def __init__(self, nxc=64, Nc=3, r=4, Nf=7, cfl=0.5, speed=1.1):
+        self.nxc, self.Nc, self.r, self.Nf = nxc, Nc, r, Nf
+        self.nxf = nxc * r
+        self.mesh_f = Mesh2D.structured(self.nxf, self.nxf, periodic=(True, True))
+        self.interp_f = Lagrange(Nf, GAUSS, Nf, GAUSS)
+        self.geom_f = SEMQuad(self.interp_f, self.mesh_f)
+        self.mesh_c = Mesh2D.structured(nxc, nxc, periodic=(True, True))
+        self.interp_c = Lagrange(Nc, GAUSS, Nc, GAUSS)
+        self.geom_c = SEMQuad(self.interp_c, self.mesh_c)
+        self.xf = self.geom_f.x.to_host()
+        self.dt_f = workloads.cfl_dt(self.geom_f.CovariantArcMin(), speed, cfl)
+        self.k = max(1, int(workloads.cfl_dt(self.geom_c.CovariantArcMin(), speed, cfl) / self.dt_f))
+        self.dt_c = self.k * self.dt_f
+        J = self.geom_c.J.to_host()
+        d = self.geom_c.dsdx.to_host()
+        self.J_c = J[:, 0, 0].copy()
+        self.metrics = np.stack([self.J_c, d[:, 0, 0, 0, 0], d[:, 0, 0, 1, 1]], axis=1)
+        self.qW_c = np.asarray(self.interp_c.qWeights, np.float64)
+        self.nIn = NVAR * (Nc + 1) ** 2 + 3
+        self.nOut = NVAR * (Nc + 1) ** 2
+
+    # --- models --------------------------------------------------------------------------
+    def fine_model(self, seed):
+        sol, topo = workloads.nonlinear_sw(self.xf, self.interp_f.controlPoints, self.interp_f.qWeights, seed=seed)
+        F = ShallowWater(self.mesh_f, self.geom_f)
+        F.SetTopography(topo)
+        F.SetSolution(sol)
+        return F, topo
+
+    def project(self, fine, nVar=NVAR):
+        """DeviceArray (coarse) of the L2 projection of a fine interior."""
+        return project_l2_structured(fine, self.Nf, self.Nc, self.nxc, self.nxc, self.r, nVar)
+
+    def coarse_model(self, topo_f):
+        topo_c = self.project(topo_f.reshape(topo_f.shape[0], 1, self.Nf + 1, self.Nf + 1), 1).to_host()
+        C = ShallowWater(self.mesh_c, self.geom_c)
+        C.SetTopography(topo_c[:, 0])
+        return C
+
+    def raw_features(self, Sc):
+        return np.concatenate([Sc.reshape(Sc.shape[0], -1), self.metrics], axis=1)
+
+    def l2(self, S):
+        """L2 norm over the coarse mesh, per variable: sqrt(sum_e J_e sum_ij w_i w_j S^2)."""
+        w2 = np.outer(self.qW_c, self.qW_c)
+        return np.sqrt(np.einsum("e,ji,evji->v", self.J_c, w2, S * S))
+
+    def mass(self, S):
+        w2 = np.outer(self.qW_c, self.qW_c)
+        return float(np.einsum("e,ji,eji->", self.J_c, w2, S[:, 2]))
+
+
+def generate(cf, seeds, n_steps=500, stride=20, log=None):
+    """Training pairs (raw features, raw targets) from fine runs of the given seeds: one
+    sample per element at every `stride`-th coarse step of an n_steps horizon."""
+    X, T = [], []
+    for seed in seeds:
+        F, topo = cf.fine_model(seed)
+        C = cf.coarse_model(topo)
+        for step in range(n_steps):
+            sample = step % stride == 0
+            if sample:
+                P0 = cf.project(F.solution_device()).to_host()
+            F.ForwardStep(cf.k, cf.dt_f)
+            if sample:
+                P1 = cf.project(F.solution_device()).to_host()
+                C.SetSolution(P0)
+                C.ForwardStep(1, cf.dt_c)
+                Sc = C.solution
+                X.append(cf.raw_features(P0))
+                T.append(((P1 - Sc) / cf.dt_c).reshape(P0.shape[0], -1))
+        if log:
+            log(f"closure data: seed {seed} done ({len(X)} snapshots)")
+        del F, C
+    return np.concatenate(X), np.concatenate(T)
+
+
+class Normalisation:
+    """Per-feature / per-output mean and scale (FP32), scale 1 where a column is constant."""
+
+    def __init__(self, X, T):
+        self.inMean = X.mean(0).astype(np.float32)
+        s = X.std(0)
+        self.inScale = np.where(s > 1e-12 * np.maximum(1.0, np.abs(X.mean(0))), s, 1.0).astype(np.float32)
+        self.outMean = T.mean(0).astype(np.float32)
+        s = T.std(0)
+        self.outScale = np.where(s > 0, s, 1.0).astype(np.float32)
+
+    def x(self, X):
+        return ((X - self.inMean.astype(np.float64)) / self.inScale.astype(np.float64)).astype(np.float32)
+
+    def t(self, T):
+        return ((T - self.outMean.astype(np.float64)) / self.outScale.astype(np.float64)).astype(np.float32)
+
+    def arrays(self):
+        return self.inMean, self.inScale, self.outMean, self.outScale
+
+    def save(self, path, params):
+        np.savez(path, inMean=self.inMean, inScale=self.inScale, outMean=self.outMean, outScale=self.outScale,
+                 **{k: np.asarray(v, np.float32) for k, v in params.items()})
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)  # plain arrays, no pickled objects
+        n = object.__new__(cls)
+        n.inMean, n.inScale, n.outMean, n.outScale = (z[k] for k in ("inMean", "inScale", "outMean", "outScale"))
+        params = {k: z[k] for k in ("W1", "b1", "W2", "b2", "W3", "b3")}
+        return n, params
+
+
+def init_params(nIn, nHidden, nOut, seed=0):
+    """He-style uniform initialisation, FP32."""
+    rng = np.random.default_rng(seed)
+
+    def lin(a, b):
+        bound = np.sqrt(6.0 / a) * 0.5
+        return rng.uniform(-bound, bound, size=(a, b)).astype(np.float32), np.zeros(b, np.float32)
+
+    W1, b1 = lin(nIn, nHidden)
+    W2, b2 = lin(nHidden, nHidden)
+    W3, b3 = lin(nHidden, nOut)
+    return {"W1": W1, "b1": b1, "W2": W2, "b2": b2, "W3": W3, "b3": b3}
+
+
+def train(mlp, Xn, Tn, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, log=None):
+    """Adam on mean-squared error over normalised (Xn, Tn) (FP32), with a geometric learning
+    rate decay from lr to lr_final; one shuffle up front, chunk order shuffled per epoch.
+    Returns the per-epoch mean training loss."""
+    rng = np.random.default_rng(seed)
+    n = Xn.shape[0]
+    perm = rng.permutation(n)
+    Xd = DeviceArray.from_host(Xn[perm])
+    Td = DeviceArray.from_host(Tn[perm])
+    offs = np.arange(0, n - batch + 1, batch)
+    total = epochs * len(offs)
+    step, hist = 0, []
+    for ep in range(epochs):
+        losses = []
+        for o in rng.permutation(offs):
+            step += 1
+            rate = lr * (lr_final / lr) ** ((step - 1) / max(1, total - 1))
+            want = len(losses) < 8  # a few synchronised loss reads per epoch
+            loss = mlp.loss_grad(Xd, Td, batch, int(o), want_loss=want)
+            if want:
+                losses.append(loss)
+            mlp.adam_step(rate, step)
+        hist.append(float(np.mean(losses)))
+        if log:
+            log(f"closure train: epoch {ep + 1}/{epochs} loss {hist[-1]:.4e}")
+    return hist
+
+
+def run_coarse(cf, C, P0, n_steps, checkpoints):
+    """Advance coarse model C from P0; states at the checkpoints and the timed milliseconds."""
+    C.SetSolution(P0)
+    C.t = 0.0
+    states, ms, done = {}, 0.0, 0
+    for c in checkpoints:
+        ms += C.time_steps(c - done, cf.dt_c)
+        done = c
+        states[c] = C.solution
+    return states, ms
+
+
+def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, log=None):
+    """Fine truth, plain coarse and (if given) closure-corrected coarse runs of one seed.
+    Returns per-checkpoint L2 errors (summed over variables) against the projected fine
+    state, mass drifts and timed milliseconds."""
+    checkpoints = checkpoints or [n_steps // 5 * (i + 1) for i in range(5)]
+    F, topo = cf.fine_model(seed)
+    C = cf.coarse_model(topo)
+    P0 = cf.project(F.solution_device()).to_host()
+    truth, fine_ms, done = {}, 0.0, 0
+    for c in checkpoints:
+        fine_ms += F.time_steps((c - done) * cf.k, cf.dt_f)
+        done = c
+        truth[c] = cf.project(F.solution_device()).to_host()
+    out = {"checkpoints": checkpoints, "fine_ms": fine_ms, "mass0": cf.mass(P0)}
+    plain, out["plain_ms"] = run_coarse(cf, C, P0, n_steps, checkpoints)
+    out["plain_err"] = [float(np.sum(cf.l2(plain[c] - truth[c]))) for c in checkpoints]
+    out["plain_mass_drift"] = (cf.mass(plain[checkpoints[-1]]) - out["mass0"]) / out["mass0"]
+    if mlp is not None:
+        C.SetClosure(mlp, *norm.arrays())
+        corr, out["closure_ms"] = run_coarse(cf, C, P0, n_steps, checkpoints)
+        C.SetClosure(None)
+        out["closure_err"] = [float(np.sum(cf.l2(corr[c] - truth[c]))) for c in checkpoints]
+        out["closure_mass_drift"] = (cf.mass(corr[checkpoints[-1]]) - out["mass0"]) / out["mass0"]
+    if log:
+        log(f"closure eval seed {seed}: " + ", ".join(f"{k}={v}" for k, v in out.items() if k.endswith("err")))
+    return out

@@ -1,0 +1,513 @@
+// Learned subgrid closure (no reference counterpart): an FP32 MLP
+// (n_in -> H -> H -> n_out, GELU) trained and evaluated by the project's own kernels, the
+// fine -> coarse L2 projection that builds its training data, and the glue that turns a
+// coarse model's state into MLP inputs and the MLP outputs into a tendency correction.
+//
+// GEMMs: C[M,N] = op(A) op(B) over FP32 with FP32 accumulation, 64x64 output tiles staged
+// through LDS (16-wide k slabs), 256 work-items computing 4x4 outputs each.  The reductions
+// over the batch dimension (weight gradients) are split over blocks and summed in a fixed
+// order by a second kernel, so training is deterministic.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "closure.hpp"
+#include "../../include/self_amd.h"
+
+using namespace selfamd;
+
+namespace {
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad_f(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+// C = op(A) * op(B) (+ bias[col]); op(A) is M x K, op(B) is K x N; lda/ldb are the row
+// strides of the stored A/B.  EPI: 0 none, 1 bias, 2 bias + GELU (Z written to Zout too).
+// Rows [kBegin, kEnd) of the reduction only (split-K); output to C + split*M*N when split.
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(const float *__restrict__ A, const float *__restrict__ B,
+                                              float *__restrict__ C, float *__restrict__ Zout,
+                                              const float *__restrict__ bias, int M, int N, int K, int lda, int ldb,
+                                              int kChunk) {
+  __shared__ float sA[TK][TM + 4];
+  __shared__ float sB[TK][TN + 4];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int row0 = blockIdx.y * TM, col0 = blockIdx.x * TN;
+  const int kBegin = blockIdx.z * kChunk, kEnd = min(K, kBegin + kChunk);
+  float acc[4][4] = {{0}};
+  for (int k0 = kBegin; k0 < kEnd; k0 += TK) {
+    for (int t = threadIdx.x; t < TK * TM; t += 256) {
+      // consecutive work-items read consecutive addresses of the stored A
+      const int kk = TA ? t / TM : t % TK, r = TA ? t % TM : t / TK;
+      const int gr = row0 + r, gk = k0 + kk;
+      float v = 0.f;
+      if (gr < M && gk < kEnd) v = TA ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk];
+      sA[kk][r] = v;
+    }
+    for (int t = threadIdx.x; t < TK * TN; t += 256) {
+      const int kk = TB ? t % TK : t / TN, c = TB ? t / TK : t % TN;
+      const int gc = col0 + c, gk = k0 + kk;
+      float v = 0.f;
+      if (gc < N && gk < kEnd) v = TB ? B[(size_t)gc * ldb + gk] : B[(size_t)gk * ldb + gc];
+      sB[kk][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk++) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) { a[q] = sA[kk][ty * 4 + q]; b[q] = sB[kk][tx * 4 + q]; }
+#pragma unroll
+      for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[p][q] = fmaf(a[p], b[q], acc[p][q]);
+    }
+    __syncthreads();
+  }
+  float *Cs = C + (size_t)blockIdx.z * M * N;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    const int r = row0 + ty * 4 + p;
+    if (r >= M) continue;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int c = col0 + tx * 4 + q;
+      if (c >= N) continue;
+      float v = acc[p][q];
+      if (EPI >= 1) v += bias[c];
+      if (EPI == 2) {
+        Zout[(size_t)r * N + c] = v;
+        v = gelu_f(v);
+      }
+      Cs[(size_t)r * N + c] = v;
+    }
+  }
+}
+
+// out[i] = sum_s parts[s*n + i] (fixed order)
+__global__ void k_sum_splits(const float *__restrict__ parts, float *__restrict__ out, int n, int splits) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int k = 0; k < splits; k++) s += parts[(size_t)k * n + i];
+    out[i] = s;
+  }
+}
+
+__global__ void k_fill(float *__restrict__ p, float v, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+// dY = 2 (Y - T) / n and the per-block partial loss sums
+__global__ void k_mse(const float *__restrict__ Y, const float *__restrict__ T, float *__restrict__ dY,
+                      float *__restrict__ partial, size_t n) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const float r = Y[t] - T[t];
+    dY[t] = 2.f * r / (float)n;
+    s += r * r;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// D = G .* gelu'(Z)
+__global__ void k_gelu_back(float *__restrict__ D, const float *__restrict__ Z, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    D[t] = D[t] * gelu_grad_f(Z[t]);
+}
+
+// Adam (Kingma & Ba 2015) with bias correction
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const float gt = g[t];
+    const float mt = b1 * m[t] + (1.f - b1) * gt;
+    const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+    m[t] = mt;
+    v[t] = vt;
+    p[t] = p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+  }
+}
+
+}  // namespace
+
+struct selfamd_mlp {
+  int nIn = 0, nH = 0, nOut = 0, maxB = 0;
+  size_t nParams = 0;
+  float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;  // packed W1 b1 W2 b2 W3 b3
+  float *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr, *dY = nullptr;
+  float *D2 = nullptr, *D1 = nullptr, *parts = nullptr, *lossParts = nullptr;
+  float *ones = nullptr;  // maxB ones: bias gradients are ones^T D through the split GEMM
+  int splitsMax = 64;
+  float *W1() { return params; }
+  float *b1() { return params + (size_t)nIn * nH; }
+  float *W2() { return b1() + nH; }
+  float *b2() { return W2() + (size_t)nH * nH; }
+  float *W3() { return b2() + nH; }
+  float *b3() { return W3() + (size_t)nH * nOut; }
+  size_t off_b1() const { return (size_t)nIn * nH; }
+  size_t off_W2() const { return off_b1() + nH; }
+  size_t off_b2() const { return off_W2() + (size_t)nH * nH; }
+  size_t off_W3() const { return off_b2() + nH; }
+  size_t off_b3() const { return off_W3() + (size_t)nH * nOut; }
+};
+
+namespace {
+
+template <bool TA, bool TB, int EPI>
+hipError_t gemm(const float *A, const float *B, float *C, float *Z, const float *bias, int M, int N, int K, int lda,
+                int ldb, int splits, hipStream_t st) {
+  const int kChunk = ((K + splits - 1) / splits + TK - 1) / TK * TK;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, 256, 0, st, A, B, C, Z, bias, M, N, K, lda, ldb, kChunk);
+  return hipGetLastError();
+}
+
+hipError_t forward_impl(selfamd_mlp *m, const float *X, float *Y, int B, hipStream_t st) {
+  hipError_t e;
+  if ((e = gemm<false, false, 2>(X, m->W1(), m->H1, m->Z1, m->b1(), B, m->nH, m->nIn, m->nIn, m->nH, 1, st))) return e;
+  if ((e = gemm<false, false, 2>(m->H1, m->W2(), m->H2, m->Z2, m->b2(), B, m->nH, m->nH, m->nH, m->nH, 1, st))) return e;
+  return gemm<false, false, 1>(m->H2, m->W3(), Y, nullptr, m->b3(), B, m->nOut, m->nH, m->nH, m->nOut, 1, st);
+}
+
+// weight gradient gW[in, out] = A^T D over B rows, split over blocks then summed in order
+hipError_t weight_grad(selfamd_mlp *m, const float *A, const float *D, float *gW, int B, int nin, int nout,
+                       hipStream_t st) {
+  int splits = (B + 255) / 256;
+  if (splits > m->splitsMax) splits = m->splitsMax;
+  if (splits < 1) splits = 1;
+  hipError_t e = gemm<true, false, 0>(A, D, m->parts, nullptr, nullptr, nin, nout, B, nin, nout, splits, st);
+  if (e) return e;
+  const int n = nin * nout;
+  hipLaunchKernelGGL(k_sum_splits, (n + 255) / 256, 256, 0, st, m->parts, gW, n, splits);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int selfamd_mlp_create(selfamd_mlp **out, int nIn, int nHidden, int nOut, int maxBatch) {
+  SELFAMD_REQUIRE(out && nIn > 0 && nHidden > 0 && nOut > 0 && maxBatch > 0, "mlp_create: bad sizes");
+  *out = nullptr;
+  selfamd_mlp *m = new selfamd_mlp();
+  m->nIn = nIn; m->nH = nHidden; m->nOut = nOut; m->maxB = maxBatch;
+  m->nParams = (size_t)nIn * nHidden + nHidden + (size_t)nHidden * nHidden + nHidden + (size_t)nHidden * nOut + nOut;
+  const size_t B = maxBatch, H = nHidden;
+  const size_t maxW = std::max({(size_t)nIn * H, H * H, H * (size_t)nOut});
+  float **bufs[] = {&m->params, &m->grads, &m->m1, &m->m2};
+  bool ok = true;
+  for (float **b : bufs) ok = ok && hipMalloc(b, 4 * m->nParams) == hipSuccess;
+  ok = ok && hipMalloc(&m->Z1, 4 * B * H) == hipSuccess && hipMalloc(&m->H1, 4 * B * H) == hipSuccess &&
+       hipMalloc(&m->Z2, 4 * B * H) == hipSuccess && hipMalloc(&m->H2, 4 * B * H) == hipSuccess &&
+       hipMalloc(&m->Y, 4 * B * nOut) == hipSuccess && hipMalloc(&m->dY, 4 * B * nOut) == hipSuccess &&
+       hipMalloc(&m->D2, 4 * B * H) == hipSuccess && hipMalloc(&m->D1, 4 * B * H) == hipSuccess &&
+       hipMalloc(&m->parts, 4 * maxW * m->splitsMax) == hipSuccess &&
+       hipMalloc(&m->lossParts, 4 * 1024) == hipSuccess && hipMalloc(&m->ones, 4 * B) == hipSuccess;
+  if (ok) ok = hipMemset(m->m1, 0, 4 * m->nParams) == hipSuccess && hipMemset(m->m2, 0, 4 * m->nParams) == hipSuccess &&
+               hipMemset(m->params, 0, 4 * m->nParams) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_fill, (unsigned)std::min<size_t>((B + 255) / 256, 4096), 256, 0, nullptr, m->ones, 1.0f, B);
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+  }
+  if (!ok) {
+    set_error("mlp_create: device allocation failed");
+    selfamd_mlp_destroy(m);
+    return kHipError;
+  }
+  *out = m;
+  return kOk;
+}
+
+int selfamd_mlp_destroy(selfamd_mlp *m) {
+  if (!m) return kOk;
+  float *bufs[] = {m->params, m->grads, m->m1, m->m2, m->Z1, m->H1, m->Z2, m->H2, m->Y, m->dY, m->D2, m->D1,
+                   m->parts, m->lossParts, m->ones};
+  for (float *b : bufs)
+    if (b) (void)hipFree(b);
+  delete m;
+  return kOk;
+}
+
+int selfamd_mlp_num_params(selfamd_mlp *m, size_t *n) {
+  SELFAMD_REQUIRE(m && n, "mlp_num_params: null argument");
+  *n = m->nParams;
+  return kOk;
+}
+
+// packed host layout: W1[nIn][H] b1[H] W2[H][H] b2[H] W3[H][nOut] b3[nOut] (row-major W[in][out])
+int selfamd_mlp_set_params(selfamd_mlp *m, const float *host) {
+  SELFAMD_REQUIRE(m && host, "mlp_set_params: null argument");
+  SELFAMD_HIP(hipMemcpy(m->params, host, 4 * m->nParams, hipMemcpyHostToDevice));
+  SELFAMD_HIP(hipMemset(m->m1, 0, 4 * m->nParams));
+  SELFAMD_HIP(hipMemset(m->m2, 0, 4 * m->nParams));
+  return kOk;
+}
+
+int selfamd_mlp_get_params(selfamd_mlp *m, float *host) {
+  SELFAMD_REQUIRE(m && host, "mlp_get_params: null argument");
+  SELFAMD_HIP(hipMemcpy(host, m->params, 4 * m->nParams, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int selfamd_mlp_get_grads(selfamd_mlp *m, float *host) {
+  SELFAMD_REQUIRE(m && host, "mlp_get_grads: null argument");
+  SELFAMD_HIP(hipMemcpy(host, m->grads, 4 * m->nParams, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int selfamd_mlp_forward(selfamd_mlp *m, const float *X, float *Y, int B, void *stream) {
+  SELFAMD_REQUIRE(m && X && Y && B >= 1 && B <= m->maxB, "mlp_forward: bad arguments (B=%d, max %d)", B, m ? m->maxB : 0);
+  SELFAMD_HIP(forward_impl(m, X, Y, B, reinterpret_cast<hipStream_t>(stream)));
+  return kOk;
+}
+
+// Forward + backward of the mean-squared error against T; gradients stay on the device
+// (selfamd_mlp_get_grads) and *loss is the FP32 loss (synchronises the stream).
+int selfamd_mlp_loss_grad(selfamd_mlp *m, const float *X, const float *T, int B, float *loss, void *stream) {
+  SELFAMD_REQUIRE(m && X && T && B >= 1 && B <= m->maxB, "mlp_loss_grad: bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int H = m->nH, nO = m->nOut;
+  SELFAMD_HIP(forward_impl(m, X, m->Y, B, st));
+  const size_t n = (size_t)B * nO;
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_mse, blocks, 256, 0, st, m->Y, T, m->dY, m->lossParts, n);
+  SELFAMD_HIP(hipGetLastError());
+  float *g = m->grads;
+  // layer 3
+  SELFAMD_HIP(weight_grad(m, m->H2, m->dY, g + m->off_W3(), B, H, nO, st));
+  SELFAMD_HIP(weight_grad(m, m->ones, m->dY, g + m->off_b3(), B, 1, nO, st));
+  SELFAMD_HIP((gemm<false, true, 0>(m->dY, m->W3(), m->D2, nullptr, nullptr, B, H, nO, nO, nO, 1, st)));
+  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D2, m->Z2,
+                     (size_t)B * H);
+  // layer 2
+  SELFAMD_HIP(weight_grad(m, m->H1, m->D2, g + m->off_W2(), B, H, H, st));
+  SELFAMD_HIP(weight_grad(m, m->ones, m->D2, g + m->off_b2(), B, 1, H, st));
+  SELFAMD_HIP((gemm<false, true, 0>(m->D2, m->W2(), m->D1, nullptr, nullptr, B, H, H, H, H, 1, st)));
+  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D1, m->Z1,
+                     (size_t)B * H);
+  // layer 1
+  SELFAMD_HIP(weight_grad(m, X, m->D1, g, B, m->nIn, H, st));
+  SELFAMD_HIP(weight_grad(m, m->ones, m->D1, g + m->off_b1(), B, 1, H, st));
+  SELFAMD_HIP(hipGetLastError());
+  if (loss) {
+    std::vector<float> parts(blocks);
+    SELFAMD_HIP(hipMemcpyAsync(parts.data(), m->lossParts, 4 * blocks, hipMemcpyDeviceToHost, st));
+    SELFAMD_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (float p : parts) s += p;
+    *loss = (float)(s / (double)n);
+  }
+  return kOk;
+}
+
+int selfamd_mlp_adam_step(selfamd_mlp *m, float lr, int step, void *stream) {
+  SELFAMD_REQUIRE(m && step >= 1, "mlp_adam_step: bad arguments");
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, (int)std::min<size_t>((m->nParams + 255) / 256, 4096), 256, 0,
+                     reinterpret_cast<hipStream_t>(stream), m->params, m->grads, m->m1, m->m2, m->nParams, lr, b1, b2,
+                     1e-8f, c1, c2);
+  SELFAMD_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------
+// Fine -> coarse L2 projection on structured meshes
+// ------------------------------------------------------------------------------
+// A coarse element (degree Nc) covers r x r fine elements (degree Nf).  Sub-element a of the
+// coarse xi-interval spans [-1 + 2a/r, -1 + 2(a+1)/r], so xi_c = -1 + (2a + 1 + xi_f)/r.  The
+// L2 projection onto the coarse Gauss-Lagrange basis, with the coarse mass matrix diagonal
+// (exact: Gauss with Nc+1 points integrates l_i l_j of degree 2 Nc), is
+//   u_c(i,j) = sum_{a,b,p,q} Px[a][i][p] Px[b][j][q] u_f(p,q; a,b),
+//   Px[a][i][p] = w_p^f l_i^c(xi_c(a,p)) / (r w_i^c),
+// exact whenever the fine quadrature integrates l_i^c times the fine polynomial (degree
+// Nc + Nf <= 2 Nf + 1), and mass-conserving: sum_ij w_i w_j u_c = r^-2 sum_ab sum_pq w_p w_q u_f.
+namespace {
+
+__global__ void k_project(const double *__restrict__ Sf, double *__restrict__ Sc, const double *__restrict__ Px, int Nf,
+                          int Nc, int nxc, int nyc, int r, int nVar) {
+  const int Pf = Nf + 1, Pc = Nc + 1;
+  const size_t total = (size_t)Pc * Pc * nVar * nxc * nyc;
+  const int nxf = nxc * r;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = t % Pc, j = (t / Pc) % Pc, v = (t / ((size_t)Pc * Pc)) % nVar;
+    const size_t E = t / ((size_t)Pc * Pc * nVar);
+    const int I = E % nxc, J = E / nxc;
+    double acc = 0.0;
+    for (int b = 0; b < r; b++)
+      for (int q = 0; q < Pf; q++) {
+        const double wy = Px[((size_t)b * Pc + j) * Pf + q];
+        double row = 0.0;
+        for (int a = 0; a < r; a++) {
+          const size_t ef = (size_t)(I * r + a) + (size_t)nxf * (J * r + b);
+          const double *u = Sf + ((size_t)v + (size_t)nVar * ef) * Pf * Pf + (size_t)Pf * q;
+          const double *px = Px + ((size_t)a * Pc + i) * Pf;
+          for (int p = 0; p < Pf; p++) row += px[p] * u[p];
+        }
+        acc += wy * row;
+      }
+    Sc[t] = acc;
+  }
+}
+
+// features: X[e][k] = (S_k - mean_k) / scale_k, rounded to FP32
+__global__ void k_closure_features(const double *__restrict__ S, const double *__restrict__ geom,
+                                   const float *__restrict__ mu, const float *__restrict__ sd, float *__restrict__ X,
+                                   int NN, int nVar, int nEl) {
+  const int nIn = nVar * NN + 3;
+  const size_t total = (size_t)nIn * nEl;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const int k = t % nIn;
+    const size_t e = t / nIn;
+    double raw;
+    if (k < nVar * NN) {
+      raw = S[e * (size_t)nVar * NN + k];
+    } else {
+      const double *g = geom + e * kGeomStride;
+      raw = k == nVar * NN ? g[4] : (k == nVar * NN + 1 ? g[0] : g[3]);
+    }
+    X[t] = (float)((raw - (double)mu[k]) / (double)sd[k]);
+  }
+}
+
+// correction: corr = Y * outScale + outMean (FP64), one wavefront per element; lane = node.
+// elemMass[e] = J_e sum_ij w_i w_j corr_mass(i,j), reduced across the wavefront in a fixed order.
+__global__ void __launch_bounds__(256) k_closure_correct(const float *__restrict__ Y, const float *__restrict__ muOut,
+                                                         const float *__restrict__ sdOut, const double *__restrict__ geom,
+                                                         const double *__restrict__ qW, double *__restrict__ corr,
+                                                         double *__restrict__ elemMass, int N, int nVar, int massVar,
+                                                         int nEl) {
+  const int P = N + 1, NN = P * P;
+  const int lane = threadIdx.x & 63;
+  const size_t e = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= (size_t)nEl) return;  // whole wavefront leaves together
+  double m = 0.0;
+  if (lane < NN) {
+    const int i = lane % P, j = lane / P;
+    for (int v = 0; v < nVar; v++) {
+      const int k = v * NN + lane;
+      const double c = (double)Y[e * (size_t)nVar * NN + k] * (double)sdOut[k] + (double)muOut[k];
+      corr[e * (size_t)nVar * NN + k] = c;
+      if (v == massVar) m = qW[i] * qW[j] * c;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off, 64);
+  if (lane == 0) elemMass[e] = geom[e * kGeomStride + 4] * m;
+}
+
+// shift = (sum_e elemMass[e]) / area, summed in a fixed order by one block
+__global__ void __launch_bounds__(256) k_closure_shift(const double *__restrict__ elemMass, double *__restrict__ shift,
+                                                       double invArea, int nEl) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < nEl; e += 256) s += elemMass[e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) shift[0] = red[0] * invArea;
+}
+
+double lagrange_basis(const double *x, int n, int i, double s) {
+  double l = 1.0;
+  for (int k = 0; k <= n; k++)
+    if (k != i) l *= (s - x[k]) / (x[i] - x[k]);
+  return l;
+}
+
+}  // namespace
+
+namespace selfamd {
+
+hipError_t mlp_forward(selfamd_mlp *m, const float *X, float *Y, int B, hipStream_t st) {
+  return forward_impl(m, X, Y, B, st);
+}
+
+int mlp_shape(const selfamd_mlp *m, int *nIn, int *nOut, int *maxB) {
+  *nIn = m->nIn;
+  *nOut = m->nOut;
+  *maxB = m->maxB;
+  return kOk;
+}
+
+hipError_t closure_features(const double *S, const double *geom, const float *inMean, const float *inScale, float *X,
+                            int N, int nVar, int nEl, hipStream_t st) {
+  const int NN = (N + 1) * (N + 1);
+  const size_t total = (size_t)(nVar * NN + 3) * nEl;
+  hipLaunchKernelGGL(k_closure_features, (unsigned)std::min<size_t>((total + 255) / 256, 65535), 256, 0, st, S, geom,
+                     inMean, inScale, X, NN, nVar, nEl);
+  return hipGetLastError();
+}
+
+hipError_t closure_evaluate(const ClosureBuffers &c, const double *S, const double *geom, const double *qW, int N,
+                            int nVar, int massVar, int nEl, hipStream_t st) {
+  hipError_t e = closure_features(S, geom, c.inMean, c.inScale, c.X, N, nVar, nEl, st);
+  if (e != hipSuccess) return e;
+  if ((e = forward_impl(c.mlp, c.X, c.Y, nEl, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_closure_correct, (nEl + 3) / 4, 256, 0, st, c.Y, c.outMean, c.outScale, geom, qW, c.corr,
+                     c.elemMass, N, nVar, massVar, nEl);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_closure_shift, 1, 256, 0, st, c.elemMass, c.shift, c.invArea, nEl);
+  return hipGetLastError();
+}
+
+}  // namespace selfamd
+
+extern "C" {
+
+int selfamd_project_l2_structured_2d(const double *fine, double *coarse, int Nf, int Nc, int nxc, int nyc, int r,
+                                     int nVar, void *stream) {
+  SELFAMD_REQUIRE(fine && coarse, "project_l2_structured_2d: null array");
+  SELFAMD_REQUIRE(Nf >= 1 && Nf <= kMaxN && Nc >= 1 && Nc <= kMaxN && nxc >= 1 && nyc >= 1 && r >= 1 && nVar >= 1,
+                  "project_l2_structured_2d: bad sizes (Nf=%d Nc=%d nxc=%d nyc=%d r=%d nVar=%d)", Nf, Nc, nxc, nyc, r,
+                  nVar);
+  SELFAMD_REQUIRE(Nc <= 2 * Nf + 1, "project_l2_structured_2d: fine quadrature too coarse for Nc=%d", Nc);
+  double xf[kMaxN + 1], tf[kMaxN + 1], bwf[kMaxN + 1], wf[kMaxN + 1];
+  double xc[kMaxN + 1], tc[kMaxN + 1], bwc[kMaxN + 1], wc[kMaxN + 1];
+  std::vector<double> iM((kMaxN + 1) * (kMaxN + 1)), dM(iM.size()), dgM(iM.size()), bM(2 * (kMaxN + 1));
+  int rc = selfamd_lagrange_init(Nf, SELFAMD_GAUSS, Nf, SELFAMD_GAUSS, xf, tf, bwf, wf, iM.data(), dM.data(),
+                                 dgM.data(), bM.data());
+  if (rc) return rc;
+  rc = selfamd_lagrange_init(Nc, SELFAMD_GAUSS, Nc, SELFAMD_GAUSS, xc, tc, bwc, wc, iM.data(), dM.data(), dgM.data(),
+                             bM.data());
+  if (rc) return rc;
+  const int Pf = Nf + 1, Pc = Nc + 1;
+  std::vector<double> px((size_t)r * Pc * Pf);
+  for (int a = 0; a < r; a++)
+    for (int i = 0; i < Pc; i++)
+      for (int p = 0; p < Pf; p++) {
+        const double xi = -1.0 + (2.0 * a + 1.0 + xf[p]) / r;
+        px[((size_t)a * Pc + i) * Pf + p] = wf[p] * lagrange_basis(xc, Nc, i, xi) / (r * wc[i]);
+      }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  double *dPx = nullptr;
+  SELFAMD_HIP(hipMalloc(&dPx, sizeof(double) * px.size()));
+  hipError_t e = hipMemcpyAsync(dPx, px.data(), sizeof(double) * px.size(), hipMemcpyHostToDevice, st);
+  const size_t total = (size_t)Pc * Pc * nVar * nxc * nyc;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_project, (unsigned)std::min<size_t>((total + 255) / 256, 65535), 256, 0, st, fine, coarse, dPx,
+                       Nf, Nc, nxc, nyc, r, nVar);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(dPx);
+  SELFAMD_HIP(e);
+  return kOk;
+}
+
+}  // extern "C"

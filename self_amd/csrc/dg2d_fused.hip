@@ -22,9 +22,10 @@
 // for the structured and block meshes); each element carries one 20-double record.
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
-#include "selfamd_common.hpp"
+#include "closure.hpp"
 #include "../../include/self_amd.h"
 
 using namespace selfamd;
@@ -32,7 +33,6 @@ using namespace selfamd;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGeomStride = 20;  // dsdx(4) J(1) dt(1, batched runs) pad(2) nx(4) ny(4) nScale(4)
 constexpr int kNbrOrphan = -0x40000000;
 
 // Williamson low-storage RK3 (src/SELF_Model.f90:25-27)
@@ -48,7 +48,11 @@ struct FusedArgs {
   double *S, *G, *dSdt;
   const double *traceIn;
   double *traceOut;
-  double *un;               // SW lagged normal velocities (0:N,2,4,nEl)
+  // SW lagged normal velocity u.n of each face node, own side only, (0:N,4,nEl): read from
+  // the previous stage's buffer (the neighbour's value, negated, is the outer side) and
+  // written to the other buffer for the next stage
+  const double *unIn;
+  double *unOut;
   const double *aux;        // SW topography / LSW bathymetry / Euler potential, or null
   const double *stat2;      // LSW Coriolis / Euler drag, or null
   const double *presTrace;  // Euler prescribed traces (0:N,9,4,nEl), or null
@@ -60,6 +64,8 @@ struct FusedArgs {
   double rkA, rkGdt;        // stage coefficients a_m and g_m*dt
   double rkG;               // g_m alone: batched runs take dt from their element records
   int perElemDt;            // 1: g_m * dt_e with dt_e in the element record (batched runs)
+  const double *corr;       // learned tendency correction (interior layout), or null
+  const double *corrShift;  // global mean of the mass-variable correction (one double)
   Params p;
 };
 
@@ -243,8 +249,13 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
 // RK register and static field on the node lanes, and (on the face lanes) the neighbour
 // code, the neighbour's traces and the lagged normal velocities.  The gather of neighbour
 // traces is the only dependent load and it overlaps the volume-flux work.
-template <class Phys, int N, int MODE, bool NODAL, int THREADS>
-__global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
+//
+// Tile parameters: THREADS work-items per block (THREADS / (N+1)^2 elements), WPE the
+// occupancy the register allocator must allow (waves per SIMD; 1 = unconstrained), XCD an
+// element order in which the blocks one XCD receives (the dispatcher deals blocks to the 8
+// XCDs in turn) cover a contiguous run of elements, so that neighbours share that XCD's L2.
+template <class Phys, int N, int MODE, bool NODAL, int THREADS, int WPE = 1, bool XCD = false>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_fused(const FusedArgs a) {
   constexpr int NP = N + 1, NN = NP * NP, EPB = THREADS / NN, NF = 4 * NP;
   constexpr int NV = Phys::NV, NT = Phys::NT;
   constexpr bool USE_AUX = Phys::kAuxGrad || Phys::kAuxTrace;
@@ -263,7 +274,12 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
   const int tid = threadIdx.x;
   const int el = tid / NN, node = tid % NN;
   const int i = node % NP, j = node / NP;
-  const int e = blockIdx.x * EPB + el;
+  int blk = blockIdx.x;
+  if constexpr (XCD) {
+    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, x = blk & 7, k = blk >> 3;
+    blk = x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
+  }
+  const int e = blk * EPB + el;
   const bool nodeActive = el < EPB && e < a.nEl;
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
 
@@ -297,7 +313,7 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
   // ---- loads: face lanes (neighbour traces, lagged velocities, normals) ----
   const int fel = tid / NF, fnode = tid % NF;
   const int side = fnode / NP, fi = fnode % NP;
-  const int fe = blockIdx.x * EPB + fel;
+  const int fe = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && fel < EPB && fe < a.nEl;
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   int code = 0;
@@ -306,15 +322,13 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
     if (code >= 0) {
       const int flip = code & 1, s2 = (code >> 1) & 3;
       const size_t e2 = (size_t)(code >> 3);
+      const size_t nb = (size_t)(flip ? N - fi : fi) + NP * (s2 + 4 * e2);
       const double *tr = a.traceIn + (size_t)(flip ? N - fi : fi) + NP * (size_t)NT * (s2 + 4 * e2);
 #pragma unroll
       for (int k = 0; k < NT; k++) qR[k] = tr[(size_t)k * NP];
+      if (Phys::kLag) unR = -a.unIn[nb];
     }
-    if (Phys::kLag) {
-      const double *un = a.un + (size_t)fi + NP * 2 * (side + 4 * (size_t)fe);
-      unL = un[0];
-      unR = un[NP];
-    }
+    if (Phys::kLag) unL = a.unIn[(size_t)fi + NP * (side + 4 * (size_t)fe)];
     if constexpr (NODAL) {
       const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
       nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
@@ -381,10 +395,9 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
     }
     if (Phys::kLag) {
       // velocity traces lag one stage (src/models/SELF_ShallowWater.f90:222-237): the stored
-      // normal velocities are used now and this stage's are stored for the next one.
-      double *un = a.un + (size_t)fi_ + NP * 2 * (side_ + 4 * (size_t)fe_);
-      un[0] = (qL[0] / qL[2]) * nx_ + (qL[1] / qL[2]) * ny_;
-      un[NP] = interior ? (qR_[0] / qR_[2]) * nx_ + (qR_[1] / qR_[2]) * ny_ : 0.0;
+      // normal velocities are used now and this stage's are stored for the next one.  The
+      // outer side's u.n is the neighbour's own value with the opposite normal.
+      a.unOut[(size_t)fi_ + NP * (side_ + 4 * (size_t)fe_)] = (qL[0] / qL[2]) * nx_ + (qL[1] / qL[2]) * ny_;
     }
     double fstar[NV];
     Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, unR_, a.p, fstar);
@@ -397,7 +410,7 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
     // N = 1, 2: more face nodes than lanes; load and compute face by face
     for (int ft = tid; ft < EPB * NF; ft += THREADS) {
       const int fel2 = ft / NF, fnode2 = ft % NF, side2 = fnode2 / NP, fi2 = fnode2 % NP;
-      const int fe2 = blockIdx.x * EPB + fel2;
+      const int fe2 = blk * EPB + fel2;
       if (fe2 >= a.nEl) continue;
       double qR2[NT], unL2 = 0.0, unR2 = 0.0, nx2, ny2, nmag2;
       const int code2 = a.nbr[4 * (size_t)fe2 + side2];
@@ -406,12 +419,9 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
         const size_t e2 = (size_t)(code2 >> 3);
         const double *tr = a.traceIn + (size_t)(flip ? N - fi2 : fi2) + NP * (size_t)NT * (s2 + 4 * e2);
         for (int k = 0; k < NT; k++) qR2[k] = tr[(size_t)k * NP];
+        if (Phys::kLag) unR2 = -a.unIn[(size_t)(flip ? N - fi2 : fi2) + NP * (s2 + 4 * e2)];
       }
-      if (Phys::kLag) {
-        const double *un = a.un + (size_t)fi2 + NP * 2 * (side2 + 4 * (size_t)fe2);
-        unL2 = un[0];
-        unR2 = un[NP];
-      }
+      if (Phys::kLag) unL2 = a.unIn[(size_t)fi2 + NP * (side2 + 4 * (size_t)fe2)];
       if constexpr (NODAL) {
         const size_t fb = (size_t)fi2 + NP * (side2 + 4 * (size_t)fe2);
         nx2 = a.nnHat[2 * fb]; ny2 = a.nnHat[2 * fb + 1]; nmag2 = a.nnScale[fb];
@@ -457,7 +467,13 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
       d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) / wi +
           (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) / wj;
       d = d / jac;
-      const double dsdt = src[v] - d;
+      double dsdt = src[v] - d;
+      if (a.corr) {
+        // closure: mass correction made mean-free over the mesh (variable 2 is H / eta / rho)
+        double c = a.corr[sBase + (size_t)v * NN + node];
+        if (v == 2) c -= a.corrShift[0];
+        dsdt += c;
+      }
       if (MODE == 1) {
         a.dSdt[sBase + (size_t)v * NN + node] = dsdt;
       } else {
@@ -481,7 +497,7 @@ __global__ void __launch_bounds__(THREADS) k_fused(const FusedArgs a) {
   for (int ft = tid; ft < EPB * NF; ft += THREADS) {
     const int fel3 = ft / NF, fnode3 = ft % NF;
     const int side3 = fnode3 / NP, fi3 = fnode3 % NP;
-    const int fe3 = blockIdx.x * EPB + fel3;
+    const int fe3 = blk * EPB + fel3;
     if (fe3 >= a.nEl) continue;
     const double *bm = (side3 == 1 || side3 == 2) ? bM1 : bM0;
     const bool alongJ = (side3 == 0 || side3 == 2);
@@ -533,11 +549,11 @@ __global__ void __launch_bounds__(kThreads) k_traces(const double *S, double *tr
   }
 }
 
-// Initial SW lagged normal velocities from the current traces: at the first stage after
-// SetSolution the reference's velocity traces come from SetSolution's BoundaryInterp.
+// Initial SW lagged normal velocities (own side of each face node) from the current traces:
+// at the first stage after SetSolution the reference's velocity traces come from
+// SetSolution's BoundaryInterp.
 template <int N>
-__global__ void k_sw_un_init(const double *trace, double *un, const double *geom, const double *nnHat,
-                             const int32_t *nbr, int nEl) {
+__global__ void k_sw_un_init(const double *trace, double *un, const double *geom, const double *nnHat, int nEl) {
   constexpr int NP = N + 1;
   const size_t total = (size_t)NP * 4 * nEl;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
@@ -551,17 +567,7 @@ __global__ void k_sw_un_init(const double *trace, double *un, const double *geom
       nx = gr[8 + side]; ny = gr[12 + side];
     }
     const double *tl = trace + fi + NP * 3 * (side + 4 * e);
-    double *u = un + fi + NP * 2 * (side + 4 * e);
-    u[0] = (tl[0] / tl[2 * NP]) * nx + (tl[NP] / tl[2 * NP]) * ny;
-    const int code = nbr[4 * e + side];
-    if (code >= 0) {
-      const int flip = code & 1, s2 = (code >> 1) & 3;
-      const size_t e2 = (size_t)(code >> 3);
-      const double *tr = trace + (flip ? N - fi : fi) + NP * 3 * (s2 + 4 * e2);
-      u[NP] = (tr[0] / tr[2 * NP]) * nx + (tr[NP] / tr[2 * NP]) * ny;
-    } else {
-      u[NP] = 0.0;
-    }
+    un[t] = (tl[0] / tl[2 * NP]) * nx + (tl[NP] / tl[2 * NP]) * ny;
   }
 }
 
@@ -572,9 +578,44 @@ constexpr int tile_threads() {
   return N == 7 ? 64 : N == 6 ? 64 : N == 5 ? 128 : N == 4 ? 128 : 256;
 }
 
+// Tuned tiles: the SW stage kernel (affine metrics, RK update) at N = 3, 5, 7 comes in 12
+// variants, v = 4 t + 2 w + x with THREADS = 64 << t, WPE = w ? 5 : 1, XCD = x.  The table
+// below (per N; -1 = the default tile) is set by selfamd_tile_set, normally from the
+// autotuner's measurements.
+constexpr int kTileVariants = 12;
+int g_swTile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+int g_tileEpoch = 0;
+
+inline bool tunable(int kind, int N) { return kind == SELFAMD_MODEL_SW && (N == 3 || N == 5 || N == 7); }
+
+template <class Phys, int N>
+hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
+  constexpr int NN = (N + 1) * (N + 1);
+  switch (v) {
+#define V(IDX, T, W, X)                                                                        \
+  case IDX: {                                                                                  \
+    constexpr int EPB = T / NN;                                                                \
+    hipLaunchKernelGGL((k_fused<Phys, N, 0, false, T, W, X>), dim3((a.nEl + EPB - 1) / EPB), T, 0, st, a); \
+    return hipGetLastError();                                                                  \
+  }
+    V(0, 64, 1, false) V(1, 64, 1, true) V(2, 64, 5, false) V(3, 64, 5, true)
+    V(4, 128, 1, false) V(5, 128, 1, true) V(6, 128, 5, false) V(7, 128, 5, true)
+    V(8, 256, 1, false) V(9, 256, 1, true) V(10, 256, 5, false) V(11, 256, 5, true)
+#undef V
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <class Phys>
 struct Launcher {
   static hipError_t stage(int N, int mode, bool nodal, const FusedArgs &a, hipStream_t st) {
+    if constexpr (std::is_same<Phys, PhysSW>::value) {
+      if (mode == 0 && !nodal && N >= 1 && N <= 7 && g_swTile[N] >= 0) {
+        if (N == 3) return launch_tile<Phys, 3>(g_swTile[N], a, st);
+        if (N == 5) return launch_tile<Phys, 5>(g_swTile[N], a, st);
+        if (N == 7) return launch_tile<Phys, 7>(g_swTile[N], a, st);
+      }
+    }
     switch (N) {
 #define CASE(NN_)                                                                                     \
   case NN_: {                                                                                         \
@@ -611,13 +652,13 @@ struct Launcher {
   }
 };
 
-hipError_t launch_un_init(int N, const double *tr, double *un, const double *geom, const double *nnHat,
-                          const int32_t *nbr, int nEl, hipStream_t st) {
+hipError_t launch_un_init(int N, const double *tr, double *un, const double *geom, const double *nnHat, int nEl,
+                          hipStream_t st) {
   const size_t n = (size_t)(N + 1) * 4 * nEl;
   const unsigned blocks = (unsigned)((n + 255) / 256);
   switch (N) {
 #define CASE(NN_) \
-  case NN_: hipLaunchKernelGGL((k_sw_un_init<NN_>), dim3(blocks), 256, 0, st, tr, un, geom, nnHat, nbr, nEl); return hipGetLastError();
+  case NN_: hipLaunchKernelGGL((k_sw_un_init<NN_>), dim3(blocks), 256, 0, st, tr, un, geom, nnHat, nEl); return hipGetLastError();
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7)
 #undef CASE
     default: return hipErrorInvalidValue;
@@ -633,7 +674,7 @@ struct selfamd_model2d {
   int kind = 0, N = 0, nEl = 0, nVar = 0, nT = 0;
   hipStream_t stream = nullptr;
   bool ownStream = false;
-  double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un = nullptr;
+  double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un[2] = {nullptr, nullptr};
   double *geom = nullptr, *mats = nullptr, *aux = nullptr, *stat2 = nullptr, *presTrace = nullptr;
   // per-node metrics (copied from SEMQuad at creation); used when `nodal` is set
   double *ndsdx = nullptr, *nJ = nullptr, *nnHat = nullptr, *nnScale = nullptr;
@@ -645,13 +686,23 @@ struct selfamd_model2d {
   // captured RK3 step graphs, one per starting trace buffer
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   double graphDt = -1.0;
+  int graphEpoch = -1;  // tile table version the graphs were captured with
+  ClosureBuffers *closure = nullptr;
 
   hipError_t launch_stage(int mode, int stage, double dt, int traceIn, double *dSdt) {
     FusedArgs a{};
+    if (closure) {
+      hipError_t e = closure_evaluate(*closure, S, geom, mats + 2 * (N + 1) * (N + 1) + 2 * (N + 1), N, nVar, 2, nEl,
+                                      stream);
+      if (e != hipSuccess) return e;
+      a.corr = closure->corr;
+      a.corrShift = closure->shift;
+    }
     a.S = S; a.G = G; a.dSdt = dSdt;
     a.traceIn = trace[traceIn];
     a.traceOut = trace[traceIn ^ 1];
-    a.un = un; a.aux = aux; a.stat2 = stat2; a.presTrace = presTrace;
+    a.unIn = un[traceIn]; a.unOut = un[traceIn ^ 1];
+    a.aux = aux; a.stat2 = stat2; a.presTrace = presTrace;
     a.geom = geom; a.nbr = nbr; a.mats = mats; a.nEl = nEl;
     a.ndsdx = ndsdx; a.nJ = nJ; a.nnHat = nnHat; a.nnScale = nnScale;
     a.rkA = kRk3A[stage];
@@ -680,6 +731,16 @@ struct selfamd_model2d {
 };
 
 using Model = selfamd_model2d;
+
+namespace {
+void free_closure(ClosureBuffers *c) {
+  if (!c) return;
+  void *bufs[] = {c->X, c->Y, c->inMean, c->inScale, c->outMean, c->outScale, c->corr, c->elemMass, c->shift};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  delete c;
+}
+}  // namespace
 
 extern "C" {
 
@@ -792,7 +853,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
     set_error("model2d_create: device allocation failed (%zu doubles of state)", nS);
     return fail(kHipError);
   }
-  if (kind == SELFAMD_MODEL_SW && hipMalloc(&m->un, sizeof(double) * (size_t)P * 2 * 4 * nEl) != hipSuccess) {
+  if (kind == SELFAMD_MODEL_SW && (hipMalloc(&m->un[0], sizeof(double) * (size_t)P * 4 * nEl) != hipSuccess ||
+                                   hipMalloc(&m->un[1], sizeof(double) * (size_t)P * 4 * nEl) != hipSuccess)) {
     set_error("model2d_create: device allocation failed (un)");
     return fail(kHipError);
   }
@@ -807,7 +869,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
       hipMemsetAsync(m->G, 0, sizeof(double) * nS, m->stream) != hipSuccess ||
       hipMemsetAsync(m->trace[0], 0, sizeof(double) * nT, m->stream) != hipSuccess ||
       hipMemsetAsync(m->trace[1], 0, sizeof(double) * nT, m->stream) != hipSuccess ||
-      (m->un && hipMemsetAsync(m->un, 0, sizeof(double) * (size_t)P * 8 * nEl, m->stream) != hipSuccess) ||
+      (m->un[0] && hipMemsetAsync(m->un[0], 0, sizeof(double) * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
+      (m->un[1] && hipMemsetAsync(m->un[1], 0, sizeof(double) * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
       hipStreamSynchronize(m->stream) != hipSuccess) {
     set_error("model2d_create: device initialisation failed");
     return fail(kHipError);
@@ -820,8 +883,9 @@ int selfamd_model2d_destroy(selfamd_model2d *m) {
   if (!m) return kOk;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   m->drop_graphs();
-  double *bufs[] = {m->S, m->G, m->trace[0], m->trace[1], m->un, m->geom, m->mats, m->aux, m->stat2, m->presTrace,
-                    m->ndsdx, m->nJ, m->nnHat, m->nnScale};
+  free_closure(m->closure);
+  double *bufs[] = {m->S,    m->G,    m->trace[0], m->trace[1], m->un[0], m->un[1], m->geom,  m->mats,
+                    m->aux,  m->stat2, m->presTrace, m->ndsdx, m->nJ,    m->nnHat, m->nnScale};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (m->nbr) (void)hipFree(m->nbr);
@@ -872,7 +936,7 @@ int selfamd_model2d_set_solution(selfamd_model2d *m, const double *interior) {
   SELFAMD_HIP(hipMemcpyAsync(m->S, interior, sizeof(double) * n, hipMemcpyDeviceToDevice, m->stream));
   SELFAMD_HIP(m->launch_traces());
   if (m->kind == SELFAMD_MODEL_SW)
-    SELFAMD_HIP(launch_un_init(m->N, m->trace[m->cur], m->un, m->geom, m->nodal ? m->nnHat : nullptr, m->nbr, m->nEl,
+    SELFAMD_HIP(launch_un_init(m->N, m->trace[m->cur], m->un[m->cur], m->geom, m->nodal ? m->nnHat : nullptr, m->nEl,
                                m->stream));
   if (m->kind == SELFAMD_MODEL_EULER) {
     // prescribed boundary state := traces of this state (SetPrescribedSolutionFromSolution)
@@ -896,6 +960,11 @@ int selfamd_model2d_get_solution(selfamd_model2d *m, double *interior) {
 int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {
   SELFAMD_REQUIRE(m && dSdt, "model2d_tendency: null argument");
   SELFAMD_HIP(m->launch_stage(1, 0, 0.0, m->cur, dSdt));
+  if (m->un[0]) {
+    // the evaluation's velocities become the lag state of the next stage
+    SELFAMD_HIP(hipMemcpyAsync(m->un[m->cur], m->un[m->cur ^ 1], sizeof(double) * (m->N + 1) * 4 * (size_t)m->nEl,
+                               hipMemcpyDeviceToDevice, m->stream));
+  }
   SELFAMD_HIP(hipStreamSynchronize(m->stream));
   return kOk;
 }
@@ -903,7 +972,7 @@ int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {
 int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
   SELFAMD_REQUIRE(m && nSteps >= 0 && dt > 0.0, "model2d_forward_step: bad arguments (nSteps=%d dt=%g)", nSteps, dt);
   if (nSteps == 0) return kOk;
-  if (m->graphDt != dt) {
+  if (m->graphDt != dt || m->graphEpoch != g_tileEpoch) {
     m->drop_graphs();
     for (int start = 0; start < 2; start++) {
       hipGraph_t g;
@@ -922,6 +991,7 @@ int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
       SELFAMD_HIP(ie);
     }
     m->graphDt = dt;
+    m->graphEpoch = g_tileEpoch;
   }
   for (int n = 0; n < nSteps; n++) {
     SELFAMD_HIP(hipGraphLaunch(m->graph[m->cur], m->stream));
@@ -955,6 +1025,90 @@ int selfamd_model2d_time_steps(selfamd_model2d *m, int nSteps, double dt, double
   return kOk;
 }
 
+int selfamd_model2d_set_closure(selfamd_model2d *m, selfamd_mlp *mlp, const float *inMean, const float *inScale,
+                                const float *outMean, const float *outScale) {
+  SELFAMD_REQUIRE(m, "model2d_set_closure: null model");
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  m->drop_graphs();
+  free_closure(m->closure);
+  m->closure = nullptr;
+  if (!mlp) return kOk;
+  SELFAMD_REQUIRE(inMean && inScale && outMean && outScale, "model2d_set_closure: null normalisation array");
+  SELFAMD_REQUIRE(m->affine, "model2d_set_closure: the closure reads the affine element records; mesh is not affine");
+  const int P = m->N + 1, NN = P * P;
+  int nIn = 0, nOut = 0, maxB = 0;
+  mlp_shape(mlp, &nIn, &nOut, &maxB);
+  SELFAMD_REQUIRE(nIn == m->nVar * NN + 3 && nOut == m->nVar * NN,
+                  "model2d_set_closure: network shape %d -> %d does not match %d nodal values + 3 metric scalars", nIn,
+                  nOut, m->nVar * NN);
+  SELFAMD_REQUIRE(maxB >= m->nEl, "model2d_set_closure: network batch %d < %d elements", maxB, m->nEl);
+  // Sum of w_i w_j J over the mesh, for the mean-free mass correction
+  std::vector<double> rec((size_t)kGeomStride * m->nEl), mats(2 * NN + 3 * P);
+  SELFAMD_HIP(hipMemcpy(rec.data(), m->geom, sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+  SELFAMD_HIP(hipMemcpy(mats.data(), m->mats, sizeof(double) * mats.size(), hipMemcpyDeviceToHost));
+  double sw = 0.0, area = 0.0;
+  for (int i = 0; i < P; i++) sw += mats[2 * NN + 2 * P + i];
+  for (int e = 0; e < m->nEl; e++) area += rec[(size_t)kGeomStride * e + 4] * sw * sw;
+  ClosureBuffers *c = new ClosureBuffers();
+  c->mlp = mlp; c->nIn = nIn; c->nOut = nOut; c->invArea = 1.0 / area;
+  const size_t nEl = m->nEl;
+  bool ok = hipMalloc(&c->X, 4 * nEl * nIn) == hipSuccess && hipMalloc(&c->Y, 4 * nEl * nOut) == hipSuccess &&
+            hipMalloc(&c->inMean, 4 * nIn) == hipSuccess && hipMalloc(&c->inScale, 4 * nIn) == hipSuccess &&
+            hipMalloc(&c->outMean, 4 * nOut) == hipSuccess && hipMalloc(&c->outScale, 4 * nOut) == hipSuccess &&
+            hipMalloc(&c->corr, 8 * nEl * nOut) == hipSuccess && hipMalloc(&c->elemMass, 8 * nEl) == hipSuccess &&
+            hipMalloc(&c->shift, 8) == hipSuccess;
+  ok = ok && hipMemcpy(c->inMean, inMean, 4 * nIn, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(c->inScale, inScale, 4 * nIn, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(c->outMean, outMean, 4 * nOut, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(c->outScale, outScale, 4 * nOut, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    free_closure(c);
+    set_error("model2d_set_closure: device allocation or copy failed");
+    return kHipError;
+  }
+  m->closure = c;
+  return kOk;
+}
+
+int selfamd_model2d_closure_features(selfamd_model2d *m, float *X) {
+  SELFAMD_REQUIRE(m && X, "model2d_closure_features: null argument");
+  SELFAMD_REQUIRE(m->closure, "model2d_closure_features: no closure attached");
+  SELFAMD_HIP(closure_features(m->S, m->geom, m->closure->inMean, m->closure->inScale, X, m->N, m->nVar, m->nEl,
+                               m->stream));
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+int selfamd_tile_count(int kind, int N, int *count) {
+  SELFAMD_REQUIRE(count, "tile_count: null argument");
+  *count = tunable(kind, N) ? kTileVariants : 0;
+  return kOk;
+}
+
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *wavesPerEU, int *xcdOrder) {
+  SELFAMD_REQUIRE(threads && wavesPerEU && xcdOrder, "tile_info: null argument");
+  SELFAMD_REQUIRE(tunable(kind, N) && variant >= 0 && variant < kTileVariants,
+                  "tile_info: no variant %d for kind %d, N=%d", variant, kind, N);
+  *threads = 64 << (variant >> 2);
+  *wavesPerEU = (variant & 2) ? 5 : 1;
+  *xcdOrder = variant & 1;
+  return kOk;
+}
+
+int selfamd_tile_set(int kind, int N, int variant) {
+  SELFAMD_REQUIRE(tunable(kind, N), "tile_set: kind %d, N=%d has no tile variants", kind, N);
+  SELFAMD_REQUIRE(variant >= -1 && variant < kTileVariants, "tile_set: variant %d out of range", variant);
+  g_swTile[N] = variant;
+  g_tileEpoch++;
+  return kOk;
+}
+
+int selfamd_tile_get(int kind, int N, int *variant) {
+  SELFAMD_REQUIRE(variant, "tile_get: null argument");
+  *variant = tunable(kind, N) ? g_swTile[N] : -1;
+  return kOk;
+}
+
 int selfamd_model2d_get_time(selfamd_model2d *m, double *t) {
   SELFAMD_REQUIRE(m && t, "model2d_get_time: null argument");
   *t = m->t;
@@ -980,7 +1134,7 @@ int selfamd_model2d_set_time(selfamd_model2d *m, double t) {
 namespace {
 struct BatchGroup {
   int N = 0, nEl = 0;
-  double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un = nullptr, *geom = nullptr;
+  double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un[2] = {nullptr, nullptr}, *geom = nullptr;
   double *mats = nullptr, *aux = nullptr, *stat2 = nullptr, *presTrace = nullptr;
   int32_t *nbr = nullptr;
   std::vector<int> runs;
@@ -1002,7 +1156,8 @@ struct selfamd_batch {
     FusedArgs a{};
     a.S = g.S; a.G = g.G; a.dSdt = nullptr;
     a.traceIn = g.trace[tin]; a.traceOut = g.trace[tin ^ 1];
-    a.un = g.un; a.aux = g.aux; a.stat2 = g.stat2; a.presTrace = g.presTrace;
+    a.unIn = g.un[tin]; a.unOut = g.un[tin ^ 1];
+    a.aux = g.aux; a.stat2 = g.stat2; a.presTrace = g.presTrace;
     a.geom = g.geom; a.nbr = g.nbr; a.mats = g.mats; a.nEl = g.nEl;
     a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
     a.p = p;
@@ -1022,7 +1177,7 @@ int selfamd_batch_destroy(selfamd_batch *b) {
   for (auto &gx : b->graph)
     if (gx) (void)hipGraphExecDestroy(gx);
   for (auto &g : b->groups) {
-    double *bufs[] = {g.S, g.G, g.trace[0], g.trace[1], g.un, g.geom, g.mats, g.aux, g.stat2, g.presTrace};
+    double *bufs[] = {g.S, g.G, g.trace[0], g.trace[1], g.un[0], g.un[1], g.geom, g.mats, g.aux, g.stat2, g.presTrace};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
     if (g.nbr) (void)hipFree(g.nbr);
@@ -1073,7 +1228,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
     if (hipMalloc(&g.S, 8 * nS) || hipMalloc(&g.G, 8 * nS) || hipMalloc(&g.trace[0], 8 * nTr) ||
         hipMalloc(&g.trace[1], 8 * nTr) || hipMalloc(&g.geom, 8 * kGeomStride * total) ||
         hipMalloc(&g.mats, 8 * (2 * PP + 3 * P)) || hipMalloc(&g.nbr, 4 * 4 * total) ||
-        (m0->un && hipMalloc(&g.un, 8 * total * P * 8)) || (m0->aux && hipMalloc(&g.aux, 8 * total * PP)) ||
+        (m0->un[0] && (hipMalloc(&g.un[0], 8 * total * P * 4) || hipMalloc(&g.un[1], 8 * total * P * 4))) || (m0->aux && hipMalloc(&g.aux, 8 * total * PP)) ||
         (m0->stat2 && hipMalloc(&g.stat2, 8 * total * PP)) || (m0->presTrace && hipMalloc(&g.presTrace, 8 * nTr))) {
       b->groups.push_back(g);
       return fail("device allocation failed");
@@ -1091,7 +1246,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
       e = e ? e : hipMemcpy(G.S + off * PP * nV, m->S, 8 * ne * PP * nV, hipMemcpyDeviceToDevice);
       e = e ? e : hipMemcpy(G.G + off * PP * nV, m->G, 8 * ne * PP * nV, hipMemcpyDeviceToDevice);
       e = e ? e : hipMemcpy(G.trace[0] + off * P * nT * 4, m->trace[m->cur], 8 * ne * P * nT * 4, hipMemcpyDeviceToDevice);
-      if (G.un) e = e ? e : hipMemcpy(G.un + off * P * 8, m->un, 8 * ne * P * 8, hipMemcpyDeviceToDevice);
+      if (G.un[0]) e = e ? e : hipMemcpy(G.un[0] + off * P * 4, m->un[m->cur], 8 * ne * P * 4, hipMemcpyDeviceToDevice);
       if (G.aux) e = e ? e : hipMemcpy(G.aux + off * PP, m->aux, 8 * ne * PP, hipMemcpyDeviceToDevice);
       if (G.stat2) e = e ? e : hipMemcpy(G.stat2 + off * PP, m->stat2, 8 * ne * PP, hipMemcpyDeviceToDevice);
       if (G.presTrace)
@@ -1172,7 +1327,8 @@ int selfamd_batch_scatter(selfamd_batch *b) {
       SELFAMD_HIP(hipMemcpy(m->G, G.G + off * PP * nV, 8 * ne * PP * nV, hipMemcpyDeviceToDevice));
       SELFAMD_HIP(hipMemcpy(m->trace[m->cur], G.trace[b->cur] + off * P * nT * 4, 8 * ne * P * nT * 4,
                             hipMemcpyDeviceToDevice));
-      if (m->un) SELFAMD_HIP(hipMemcpy(m->un, G.un + off * P * 8, 8 * ne * P * 8, hipMemcpyDeviceToDevice));
+      if (m->un[0])
+        SELFAMD_HIP(hipMemcpy(m->un[m->cur], G.un[b->cur] + off * P * 4, 8 * ne * P * 4, hipMemcpyDeviceToDevice));
     }
   }
   return kOk;

@@ -1,4 +1,4 @@
-"""Batched runs (configuration C4): uneven meshes and degrees advanced together must give
+"""Batched runs: uneven meshes and degrees advanced together must give
 each run bit-for-bit the state it reaches when advanced alone, and match the oracle."""
 import numpy as np
 import pytest

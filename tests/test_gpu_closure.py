@@ -1,0 +1,156 @@
+"""Learned closure kernels against the CPU oracles: FP32 MLP forward, gradients and Adam
+(oracle/mlp.py, FP64), the fine -> coarse L2 projection and the closure features /
+correction (oracle/closure.py), and mass conservation of a corrected run."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import closure as oc
+from oracle import mlp as om
+
+pytestmark = pytest.mark.gpu
+
+NIN, NH, NOUT = 51, 128, 48
+
+
+def _mlp(seed=0, maxB=4096):
+    import self_amd
+    p64 = om.init_params(NIN, NH, NOUT, seed=seed)
+    p64 = {k: v + (0.05 * np.random.default_rng(seed + 1).standard_normal(v.shape) if k.startswith("b") else 0)
+           for k, v in p64.items()}
+    p32 = {k: v.astype(np.float32) for k, v in p64.items()}
+    net = self_amd.MLP(NIN, NH, NOUT, maxB)
+    net.set_params(p32)
+    return net, {k: v.astype(np.float64) for k, v in p32.items()}
+
+
+def _maxrel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("rows", [1, 300, 4096])
+def test_mlp_forward_matches_oracle(rows):
+    net, p = _mlp(0)
+    X = np.random.default_rng(rows).standard_normal((rows, NIN)).astype(np.float32)
+    y = net.forward(X)
+    want, _ = om.forward(p, X.astype(np.float64))
+    assert _maxrel(y, want) <= 2e-6
+
+
+@pytest.mark.parametrize("rows", [7, 333, 2048])
+def test_mlp_gradients_match_oracle(rows):
+    import self_amd
+    net, p = _mlp(1)
+    rng = np.random.default_rng(10 + rows)
+    X = rng.standard_normal((rows, NIN)).astype(np.float32)
+    T = rng.standard_normal((rows, NOUT)).astype(np.float32)
+    loss = net.loss_grad(self_amd.DeviceArray.from_host(X), self_amd.DeviceArray.from_host(T))
+    want_loss, want = om.loss_and_grads(p, X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - want_loss) <= 1e-5 * want_loss
+    g = net.get_grads()
+    for k in want:
+        assert _maxrel(g[k], want[k]) <= 2e-5, k
+
+
+def test_adam_matches_oracle():
+    import self_amd
+    net, p = _mlp(2)
+    rng = np.random.default_rng(3)
+    X = self_amd.DeviceArray.from_host(rng.standard_normal((256, NIN)).astype(np.float32))
+    T = self_amd.DeviceArray.from_host(rng.standard_normal((256, NOUT)).astype(np.float32))
+    m = {k: np.zeros_like(v) for k, v in p.items()}
+    v = {k: np.zeros_like(x) for k, x in p.items()}
+    for step in (1, 2, 3):
+        net.loss_grad(X, T)
+        g = {k: x.astype(np.float64) for k, x in net.get_grads().items()}
+        net.adam_step(1e-3, step)
+        om.adam_step(p, g, m, v, step, lr=1e-3)
+        got = net.get_params()
+        for k in p:
+            assert np.max(np.abs(got[k] - p[k])) <= 1e-6, (step, k)
+        p = {k: x.astype(np.float64) for k, x in got.items()}  # continue from the FP32 state
+
+
+def test_mlp_training_reduces_loss():
+    import self_amd
+    net, _ = _mlp(4)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((8192, NIN)).astype(np.float32)
+    A = rng.standard_normal((NIN, NOUT)) / np.sqrt(NIN)
+    T = np.tanh(X @ A).astype(np.float32)
+    Xd, Td = self_amd.DeviceArray.from_host(X), self_amd.DeviceArray.from_host(T)
+    first = net.loss_grad(Xd, Td, 1024, 0)
+    for step in range(1, 401):
+        net.loss_grad(Xd, Td, 1024, 1024 * (step % 8), want_loss=False)
+        net.adam_step(2e-3, step)
+    last = net.loss_grad(Xd, Td, 1024, 0)
+    assert last < 0.1 * first
+
+
+@pytest.mark.parametrize("Nf,Nc,r,nxc,nyc", [(7, 3, 4, 3, 2), (5, 3, 2, 2, 3), (3, 3, 1, 2, 2)])
+def test_projection_matches_oracle(Nf, Nc, r, nxc, nyc):
+    import self_amd
+    rng = np.random.default_rng(Nf * r)
+    fine = rng.standard_normal((r * nxc * r * nyc, 3, Nf + 1, Nf + 1))
+    got = self_amd.project_l2_structured(fine, Nf, Nc, nxc, nyc, r, 3).to_host()
+    want = oc.project_l2_structured(fine, Nf, Nc, nxc, nyc, r)
+    assert _maxrel(got, want) <= 1e-14
+
+
+def _coarse_model(nx=8, N=3, seed=0):
+    import self_amd
+    from self_amd import workloads
+    mesh = self_amd.Mesh2D.structured(nx, nx, periodic=(True, True))
+    interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
+    geom = self_amd.SEMQuad(interp, mesh)
+    sol, topo = workloads.nonlinear_sw(geom.x.to_host(), interp.controlPoints, interp.qWeights, seed=seed,
+                                       width=0.3, vel_std=0.05)
+    m = self_amd.ShallowWater(mesh, geom)
+    m.SetTopography(topo)
+    m.SetSolution(sol)
+    J = geom.J.to_host()[:, 0, 0].copy()
+    d = geom.dsdx.to_host()
+    return m, sol, J, d[:, 0, 0, 0, 0].copy(), d[:, 0, 0, 1, 1].copy(), np.asarray(interp.qWeights)
+
+
+def _norm(seed, scale_out=0.05):
+    rng = np.random.default_rng(seed)
+    inMean = (0.1 * rng.standard_normal(NIN)).astype(np.float32)
+    inScale = (0.5 + rng.random(NIN)).astype(np.float32)
+    outMean = (0.01 * rng.standard_normal(NOUT)).astype(np.float32)
+    outScale = (scale_out * (0.5 + rng.random(NOUT))).astype(np.float32)
+    return inMean, inScale, outMean, outScale
+
+
+def test_closure_features_and_correction_match_oracle():
+    m, sol, J, d11, d22, qW = _coarse_model()
+    net, _ = _mlp(6)
+    inMean, inScale, outMean, outScale = _norm(7)
+    plain = m.CalculateTendency()
+    m.SetClosure(net, inMean, inScale, outMean, outScale)
+    X = m.closure_features()
+    want_X = oc.features(sol, J, d11, d22, inMean, inScale)
+    assert np.array_equal(X, want_X)
+    Y = net.forward(X)
+    corr = oc.correction(Y, outMean, outScale, qW, J, 3, 3)
+    got = m.CalculateTendency() - plain
+    scale = np.max(np.abs(plain))
+    assert np.max(np.abs(got - corr)) <= 1e-14 * scale + 1e-15
+    m.SetClosure(None)
+    assert np.max(np.abs(m.CalculateTendency() - plain)) <= 1e-14 * scale
+
+
+def test_closure_run_conserves_mass():
+    m, sol, J, d11, d22, qW = _coarse_model(16, 3, seed=2)
+    net, _ = _mlp(8)
+    m.SetClosure(net, *_norm(9, scale_out=0.1))
+    mass0 = oc.mass(sol, qW, J)
+    dt = 2e-3
+    m.ForwardStep(200, dt)
+    s = m.solution
+    assert np.isfinite(s).all()
+    assert abs(oc.mass(s, qW, J) - mass0) <= 1e-13 * abs(mass0)
+    # the correction did act
+    m2, *_ = _coarse_model(16, 3, seed=2)
+    m2.ForwardStep(200, dt)
+    assert np.max(np.abs(m2.solution - s)) > 1e-6
