@@ -1,17 +1,22 @@
-"""Benchmark: nonlinear shallow water 2D, 256x256 elements, N = 7, FP64, periodic, Rusanov
-flux, low-storage RK3 -- the fused DGSEM tendency + RK3 engine on one MI355X.
+"""Benchmark of the fused DGSEM tendency + low-storage RK3 engine on one MI355X.
 
-One step = one full RK3 step (3 fused stage launches) over the whole mesh with the state
-resident in HBM.  value = DOF-updates/s = DOF * steps / seconds, timed with hipEvents on the
-model's stream around graph-replayed steps (warm-up excluded).
+Default workload: nonlinear shallow water 2D, 256x256 periodic elements, N = 7 Gauss, FP64,
+Rusanov flux, RK3, 100 steps.  One step = one full RK3 step (3 fused stage launches) over
+the whole mesh with the state resident in HBM.  value = DOF-updates/s = DOF * steps /
+seconds, timed with hipEvents on the model's stream around graph-replayed steps (warm-up
+excluded).  Other workloads (--workload): euler512 (compressible Euler, 512x512 walls,
+N = 7, gravity, 50 steps), lsw10 (linear SW gravity wave, 10x10 walls, N = 7, 200 steps) and
+ensemble (512 batched SW runs, 8x8..64x64 elements, N in {3, 5, 7}).
 
-roofline: the fused stage kernel is HBM-bound; its algorithmic traffic is 120 bytes per
-DOF-update (FP64 state + RK register read/write, one face-trace write and one neighbour-trace
-read per stage, one static topography field; affine metrics), i.e. 40 bytes per DOF per stage
-launch.  achieved = 40 B * DOF / average launch duration; peak = 8 TB/s (MI355X HBM3E).
+roofline: the stage kernel is HBM-bound.  Its algorithmic traffic per DOF and RK step is
+3 (32 + 64 NT / ((N+1) nVar) + 8 p / nVar) - 8 bytes: state and RK register read and written
+each stage (no register read at the first stage), NT face-trace values per face node written
+once and read once per stage, p static per-node fields read once per stage.  For SW at N = 7
+(NT = nVar = 3, p = 1) that is 120 B per DOF-update, 40 B per DOF per stage launch.
+achieved = algorithmic bytes per launch / average launch duration; peak = 8 TB/s.
 
-cpu_baseline: the oracle (faithful C restatement of the reference Fortran CPU path) timed on
-the host for one RK3 step of the same workload with all available host threads.
+cpu_baseline: the oracle (C restatement of the reference Fortran CPU path, OpenMP over
+elements) timed on this host for a bounded number of RK3 steps of the same workload.
 """
 import argparse
 import json
@@ -24,18 +29,21 @@ import numpy as np
 METRIC = ("DOF-updates/sec + % HBM roofline, SW2D N=7 256² elems FP64; "
           "closure speedup at equal L2 error")
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_DOF_UPDATE = 120.0  # SW, N = 7, one static field
+
+
+def alg_bytes_per_dof_step(N, nVar, NT, p):
+    """Algorithmic HBM bytes per DOF and RK3 step (three stage launches)."""
+    return 3.0 * (32.0 + 64.0 * NT / ((N + 1) * nVar) + 8.0 * p / nVar) - 8.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None, help="timed RK3 steps (default per workload)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--N", type=int, default=7)
+    ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble"])
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=100, help="RK3 steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: 3 dispatches of a 1 GiB FP64 copy (calibration), "
@@ -43,9 +51,9 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(root):
+def traffic_from_profiles(root, name):
     """Measured HBM bytes per stage launch, if a counter summary has been committed."""
-    path = os.path.join(root, "profiles", "sw2d_stage_256x256_N7_traffic.json")
+    path = os.path.join(root, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -72,80 +80,211 @@ def traffic_probe(self_amd, model, dt):
     return 0
 
 
+def tile_name(self_amd, N):
+    from self_amd import autotune
+    v = autotune.get_tile(N) if autotune.count(N) else -1
+    if v < 0:
+        t = 64 if N >= 6 else (128 if N >= 4 else 256)
+        return f"k_fused<PhysSW, {N}, 0, false, {t}, 1, false>"
+    i = autotune.info(N, v)
+    return f"k_fused<PhysSW, {N}, 0, false, {i['threads']}, 1, {str(i['xcd_order']).lower()}>"
+
+
+# ---------------------------------------------------------------------------------------
+# single-model workloads
+# ---------------------------------------------------------------------------------------
+def setup_single(self_amd, workloads, name, seed):
+    if name == "sw256":
+        N, nx = 7, 256
+        mesh = self_amd.Mesh2D.structured(nx, nx, periodic=(True, True))
+    elif name == "euler512":
+        N, nx = 7, 512
+        mesh = self_amd.Mesh2D.structured(nx, nx, bcid=self_amd.BC_NONORMALFLOW)
+    else:
+        N, nx = 7, 10
+        mesh = self_amd.Mesh2D.structured(nx, nx, bcid=self_amd.BC_NONORMALFLOW)
+    interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
+    geom = self_amd.SEMQuad(interp, mesh)
+    x = geom.x.to_host()
+    arcmin = geom.CovariantArcMin()
+    if name == "sw256":
+        sol, topo = workloads.nonlinear_sw(x, interp.controlPoints, interp.qWeights, seed=seed)
+        model = self_amd.ShallowWater(mesh, geom)
+        model.SetTopography(topo)
+        H = sol[:, 2]
+        speed = float((np.sqrt(sol[:, 0] ** 2 + sol[:, 1] ** 2) / H + np.sqrt(H)).max())
+        fields, desc = {"topography": topo}, (f"nonlinear shallow water 2D, {nx}x{nx} quad elements periodic, "
+                                              f"N={N} Gauss, FP64, Rusanov flux, low-storage RK3")
+        nVar, NT, p, steps = 3, 3, 1, 100
+    elif name == "euler512":
+        sol, phi = workloads.euler_hydrostatic(x, seed=seed)
+        model = self_amd.CompressibleIdealGas2D(mesh, geom)
+        model.SetGravity(phi)
+        rho, rhoE = sol[:, 2], sol[:, 3]
+        pr = 0.4 * (rhoE - 0.5 * (sol[:, 0] ** 2 + sol[:, 1] ** 2) / rho)
+        speed = float((np.sqrt(sol[:, 0] ** 2 + sol[:, 1] ** 2) / rho + np.sqrt(1.4 * pr / rho)).max())
+        fields, desc = {"potential": phi}, (f"compressible Euler 2D, {nx}x{nx} quad elements, no-normal-flow "
+                                            f"walls, N={N} Gauss, FP64, LLF flux, gravity, low-storage RK3")
+        nVar, NT, p, steps = 4, 9, 1, 50
+    else:
+        sol = workloads.gravity_wave_lsw(x, seed=seed)
+        model = self_amd.LinearShallowWater(mesh, geom)
+        Hb = np.ones((mesh.nElem, N + 1, N + 1))
+        model.SetBathymetry(Hb)
+        speed = 1.0
+        fields, desc = {"bathymetry": Hb}, (f"linear shallow water 2D gravity wave, {nx}x{nx} quad elements, "
+                                            f"reflecting walls, N={N} Gauss, FP64, low-storage RK3")
+        nVar, NT, p, steps = 3, 3, 1, 200
+    model.SetSolution(sol)
+    dt = workloads.cfl_dt(arcmin, speed, 0.5)
+    return dict(N=N, nx=nx, mesh=mesh, geom=geom, model=model, sol=sol, fields=fields, dt=dt, desc=desc,
+                nVar=nVar, NT=NT, p=p, steps=steps)
+
+
+def cpu_single(oracle, name, w, steps):
+    threads = min(16, os.cpu_count() or 1)
+    og = oracle.Geometry(w["N"], w["mesh"].nodeCoords, 1)
+    kind = {"sw256": oracle.SW, "euler512": oracle.EULER, "lsw10": oracle.LSW}[name]
+    om = oracle.Model(kind, w["N"], w["mesh"].sideInfo, og)
+    for k, v in w["fields"].items():
+        {"topography": om.set_topography, "potential": om.set_gravity, "bathymetry": om.set_bathymetry}[k](v)
+    om.set_solution(w["sol"])
+    if name == "euler512":
+        om.set_prescribed()
+    oracle.set_threads(threads)
+    c0 = time.perf_counter()
+    om.rk3_steps(steps, w["dt"])
+    csec = time.perf_counter() - c0
+    oracle.set_threads(1)
+    dof = w["mesh"].nElem * (w["N"] + 1) ** 2 * w["nVar"]
+    return {"value": dof * steps / csec, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} RK3 step(s) of the same {w['nx']}x{w['nx']} N={w['N']} workload, "
+                      f"OpenMP over elements, {csec:.2f} s"}
+
+
+def run_single(a, root):
+    import self_amd
+    from self_amd import workloads
+    w = setup_single(self_amd, workloads, a.workload, a.seed)
+    model, dt, N = w["model"], w["dt"], w["N"]
+    if a.traffic_probe:
+        return traffic_probe(self_amd, model, dt)
+    steps = a.steps or w["steps"]
+    dof = w["mesh"].nElem * (N + 1) ** 2 * w["nVar"]
+    model.ForwardStep(a.warmup, dt)
+    self_amd.synchronize()
+    t0 = time.perf_counter()
+    ms = model.time_steps(steps, dt)
+    self_amd.synchronize()
+    wall = time.perf_counter() - t0
+    if not np.isfinite(model.solution).all():
+        print("non-finite state after the timed steps", file=sys.stderr)
+        return 1
+    sec = ms / 1e3
+    launch_s = sec / (3 * steps)
+    bpd = alg_bytes_per_dof_step(N, w["nVar"], w["NT"], w["p"])
+    achieved = bpd / 3.0 * dof / launch_s / 1e9
+    cpu = None
+    if not a.no_cpu_baseline:
+        import oracle
+        cpu = cpu_single(oracle, a.workload, w, a.cpu_steps or {"sw256": 100, "euler512": 5, "lsw10": 200}[a.workload])
+    kernel = tile_name(self_amd, N) if a.workload == "sw256" else (
+        f"k_fused<{'PhysEuler' if a.workload == 'euler512' else 'PhysLSW'}, {N}, 0, false, 64, 1, false>")
+    traffic = traffic_from_profiles(root, "sw2d_stage_256x256_N7_traffic.json") if a.workload == "sw256" else None
+    out = {
+        "metric": METRIC if a.workload == "sw256" else f"DOF-updates/sec + % HBM roofline, {a.workload}",
+        "value": dof * steps / sec, "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+                     "avg_launch_us": launch_s * 1e6, "bytes_per_launch_alg": bpd / 3.0 * dof},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+# ---------------------------------------------------------------------------------------
+# batched ensemble
+# ---------------------------------------------------------------------------------------
+def run_ensemble(a, root, n_runs=512, n_check=16):
+    import self_amd
+    from self_amd import workloads
+    rng = np.random.default_rng(a.seed)
+    sizes = rng.integers(8, 65, size=n_runs)
+    degrees = rng.choice([3, 5, 7], size=n_runs)
+    cache = {}
+    models, dts, inits = [], [], []
+    t0 = time.perf_counter()
+    for r in range(n_runs):
+        n, N = int(sizes[r]), int(degrees[r])
+        if (n, N) not in cache:
+            mesh = self_amd.Mesh2D.structured(n, n, periodic=(True, True))
+            interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
+            geom = self_amd.SEMQuad(interp, mesh)
+            cache[(n, N)] = (mesh, interp, geom, geom.x.to_host(), geom.CovariantArcMin())
+        mesh, interp, geom, x, arcmin = cache[(n, N)]
+        sol, topo = workloads.nonlinear_sw(x, interp.controlPoints, interp.qWeights, seed=1000 + r)
+        m = self_amd.ShallowWater(mesh, geom)
+        m.SetTopography(topo)
+        m.SetSolution(sol)
+        models.append(m)
+        dts.append(workloads.cfl_dt(arcmin, 1.2, 0.5))
+        inits.append((sol, topo))
+    setup_s = time.perf_counter() - t0
+    batch = self_amd.Batch(models, dts)
+    groups, dof = batch.info()
+    steps = a.steps or 20
+    batch.ForwardStep(a.warmup)
+    self_amd.synchronize()
+    ms = batch.time_steps(steps)
+    batch.scatter()
+    # per-run error against solving the run alone (same initial state, same steps)
+    errs = []
+    for r in np.linspace(0, n_runs - 1, n_check).astype(int):
+        n, N = int(sizes[r]), int(degrees[r])
+        mesh, interp, geom, x, arcmin = cache[(n, N)]
+        m = self_amd.ShallowWater(mesh, geom)
+        m.SetTopography(inits[r][1])
+        m.SetSolution(inits[r][0])
+        m.ForwardStep(a.warmup + steps, dts[r])
+        ref, got = m.solution, models[r].solution
+        errs.append(float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))))
+    bytes_step = 0.0
+    for r in range(n_runs):
+        N = int(degrees[r])
+        bytes_step += int(sizes[r]) ** 2 * (N + 1) ** 2 * 3 * alg_bytes_per_dof_step(N, 3, 3, 1)
+    sec = ms / 1e3
+    achieved = bytes_step * steps / sec / 1e9
+    out = {
+        "metric": "aggregate DOF-updates/s, 512 batched SW2D runs", "value": dof * steps / sec,
+        "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n_runs} nonlinear SW 2D runs, n x n periodic elements with n ~ U{{8..64}}, "
+                               f"N ~ U{{3,5,7}}, one fused launch per degree group and RK stage, one hipGraph "
+                               f"per step", "runs": n_runs, "groups": groups, "dof": dof, "setup_s": setup_s,
+                   "max_rel_err_vs_separate": max(errs), "runs_checked": n_check},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fused<PhysSW, N, 0, false, T> (one per degree group)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
-    import self_amd
-    from self_amd import workloads
-
     root = os.path.dirname(os.path.abspath(__file__))
-    N, nx = a.N, a.nx
-    mesh = self_amd.Mesh2D.structured(nx, nx, periodic=(True, True))
-    interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
-    geom = self_amd.SEMQuad(interp, mesh)
-    x = geom.x.to_host()
-    sol, topo = workloads.nonlinear_sw(x, interp.controlPoints, interp.qWeights, seed=a.seed)
-    model = self_amd.ShallowWater(mesh, geom)
-    model.SetTopography(topo)
-    model.SetSolution(sol)
-    H = sol[:, 2]
-    speed = float((np.sqrt(sol[:, 0] ** 2 + sol[:, 1] ** 2) / H + np.sqrt(H)).max())
-    dt = workloads.cfl_dt(geom.CovariantArcMin(), speed, 0.5)
-    dof = mesh.nElem * (N + 1) ** 2 * 3
-
-    if a.traffic_probe:
-        return traffic_probe(self_amd, model, dt)
-
-    model.ForwardStep(a.warmup, dt)
-    self_amd.synchronize()
-    t0 = time.perf_counter()
-    ms = model.time_steps(a.steps, dt)
-    self_amd.synchronize()
-    wall = time.perf_counter() - t0
-    s = model.solution
-    if not np.isfinite(s).all():
-        print("non-finite state after the timed steps", file=sys.stderr)
-        return 1
-    sec = ms / 1e3
-    value = dof * a.steps / sec
-    launch_s = sec / (3 * a.steps)
-    achieved = BYTES_PER_DOF_UPDATE / 3.0 * dof / launch_s / 1e9
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        og = oracle.Geometry(N, mesh.nodeCoords, 1)
-        om = oracle.Model(oracle.SW, N, mesh.sideInfo, og)
-        om.set_topography(topo)
-        om.set_solution(sol)
-        oracle.set_threads(threads)
-        c0 = time.perf_counter()
-        om.rk3_steps(a.cpu_steps, dt)
-        csec = time.perf_counter() - c0
-        oracle.set_threads(1)
-        cpu = {"value": dof * a.cpu_steps / csec, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{a.cpu_steps} RK3 step(s) of the same {nx}x{nx} N={N} workload, "
-                         f"OpenMP over elements, {csec:.2f} s"}
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"nonlinear shallow water 2D, {nx}x{nx} quad elements periodic, N={N} Gauss, "
-                               f"FP64, Rusanov flux, low-storage RK3", "elements": mesh.nElem, "N": N,
-                   "dof": dof, "dt": dt, "wall_s": wall},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(root),
-                     "kernel": "k_fused<PhysSW, 7, 0, false, 64>", "avg_launch_us": launch_s * 1e6,
-                     "bytes_per_launch_alg": BYTES_PER_DOF_UPDATE / 3.0 * dof},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
-    return 0
+    if a.workload == "ensemble":
+        return run_ensemble(a, root)
+    return run_single(a, root)
 
 
 if __name__ == "__main__":

@@ -290,13 +290,15 @@ int selfamd_model2d_synchronize(selfamd_model2d *model);
 /* forward_step bracketed by two hipEvents on the model's stream; returns the elapsed
  * milliseconds between them (synchronises). */
 int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, double *elapsedMs);
-/* Tile variants of the nonlinear-SW stage kernel (affine metrics) at N = 3, 5, 7: block size,
- * occupancy target and XCD-contiguous element order.  No reference counterpart; the
- * autotuner (self_amd/autotune.py) times them and records the fastest per N.  A set variant
- * (-1 = default tile) applies to every model's next graph capture and to batches created
- * afterwards; it never changes results (same arithmetic per element). */
+/* Tile variants of the nonlinear-SW stage kernel (affine metrics) at N = 3, 5, 7: block size
+ * (64, 128, 256 work-items), element order alternating between stages (so a stage starts on
+ * the data the previous one left in the Infinity Cache) and XCD-contiguous element order.
+ * No reference counterpart; the autotuner (self_amd/autotune.py) times them and records the
+ * fastest per N.  A set variant (-1 = default tile) applies to every model's next graph
+ * capture and to batches created afterwards; it never changes results (same arithmetic per
+ * element). */
 int selfamd_tile_count(int kind, int N, int *count);
-int selfamd_tile_info(int kind, int N, int variant, int *threads, int *wavesPerEU, int *xcdOrder);
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder);
 int selfamd_tile_set(int kind, int N, int variant);
 int selfamd_tile_get(int kind, int N, int *variant);
 /* Model time t */

@@ -518,3 +518,10 @@ def project_l2_structured(fine, Nf, Nc, nxc, nyc, r, nVar):
     out = DeviceArray((nxc * nyc, nVar, Nc + 1, Nc + 1))
     call("selfamd_project_l2_structured_2d", f.ptr, out.ptr, Nf, Nc, nxc, nyc, r, nVar, None)
     return out
+
+
+# Tuned tiles of the SW stage kernel (self_amd/tuning/sw_tiles.json, written by
+# `python -m self_amd.autotune`); without the table the default tiles are used.
+from . import autotune as _autotune  # noqa: E402
+
+_autotune.apply()

@@ -64,6 +64,8 @@ struct FusedArgs {
   double rkA, rkGdt;        // stage coefficients a_m and g_m*dt
   double rkG;               // g_m alone: batched runs take dt from their element records
   int perElemDt;            // 1: g_m * dt_e with dt_e in the element record (batched runs)
+  int writeG;               // 0 at the last RK3 stage: the next step's first stage (a_1 = 0) never reads G
+  int reverse;              // 1: blocks take the elements in reverse order (alternate stages)
   const double *corr;       // learned tendency correction (interior layout), or null
   const double *corrShift;  // global mean of the mass-variable correction (one double)
   Params p;
@@ -254,6 +256,9 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
 // occupancy the register allocator must allow (waves per SIMD; 1 = unconstrained), XCD an
 // element order in which the blocks one XCD receives (the dispatcher deals blocks to the 8
 // XCDs in turn) cover a contiguous run of elements, so that neighbours share that XCD's L2.
+// With a.reverse the elements are visited last to first: successive stages alternate
+// direction, so a stage starts on the elements whose data the previous stage touched last
+// and which are still held by the memory-side Infinity Cache.
 template <class Phys, int N, int MODE, bool NODAL, int THREADS, int WPE = 1, bool XCD = false>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_fused(const FusedArgs a) {
   constexpr int NP = N + 1, NN = NP * NP, EPB = THREADS / NN, NF = 4 * NP;
@@ -279,6 +284,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, x = blk & 7, k = blk >> 3;
     blk = x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
   }
+  if (a.reverse) blk = (int)gridDim.x - 1 - blk;
   const int e = blk * EPB + el;
   const bool nodeActive = el < EPB && e < a.nEl;
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
@@ -479,7 +485,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       } else {
         const double gn = a.rkA * gReg[v] + dsdt;
         s[v] = s[v] + gdt * gn;
-        a.G[sBase + (size_t)v * NN + node] = gn;
+        if (a.writeG) a.G[sBase + (size_t)v * NN + node] = gn;
         a.S[sBase + (size_t)v * NN + node] = s[v];
       }
     }
@@ -579,11 +585,14 @@ constexpr int tile_threads() {
 }
 
 // Tuned tiles: the SW stage kernel (affine metrics, RK update) at N = 3, 5, 7 comes in 12
-// variants, v = 4 t + 2 w + x with THREADS = 64 << t, WPE = w ? 5 : 1, XCD = x.  The table
-// below (per N; -1 = the default tile) is set by selfamd_tile_set, normally from the
-// autotuner's measurements.
+// variants, v = 4 t + 2 r + x with THREADS = 64 << t, element order alternating between
+// stages when r = 1, XCD-contiguous element order when x = 1.  The table below (per N; -1 =
+// the default tile) is set by selfamd_tile_set, normally from the autotuner's measurements.
+// Every variant performs the same arithmetic per element (bit-identical results).
 constexpr int kTileVariants = 12;
 int g_swTile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+// element order alternates between stages (the default tiles alternate too)
+inline bool tile_alternates(int N) { return g_swTile[N] < 0 || (g_swTile[N] & 2); }
 int g_tileEpoch = 0;
 
 inline bool tunable(int kind, int N) { return kind == SELFAMD_MODEL_SW && (N == 3 || N == 5 || N == 7); }
@@ -598,9 +607,9 @@ hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
     hipLaunchKernelGGL((k_fused<Phys, N, 0, false, T, W, X>), dim3((a.nEl + EPB - 1) / EPB), T, 0, st, a); \
     return hipGetLastError();                                                                  \
   }
-    V(0, 64, 1, false) V(1, 64, 1, true) V(2, 64, 5, false) V(3, 64, 5, true)
-    V(4, 128, 1, false) V(5, 128, 1, true) V(6, 128, 5, false) V(7, 128, 5, true)
-    V(8, 256, 1, false) V(9, 256, 1, true) V(10, 256, 5, false) V(11, 256, 5, true)
+    V(0, 64, 1, false) V(1, 64, 1, true) V(2, 64, 1, false) V(3, 64, 1, true)
+    V(4, 128, 1, false) V(5, 128, 1, true) V(6, 128, 1, false) V(7, 128, 1, true)
+    V(8, 256, 1, false) V(9, 256, 1, true) V(10, 256, 1, false) V(11, 256, 1, true)
 #undef V
     default: return hipErrorInvalidValue;
   }
@@ -708,6 +717,9 @@ struct selfamd_model2d {
     a.rkA = kRk3A[stage];
     a.rkGdt = kRk3G[stage] * dt;
     a.rkG = kRk3G[stage];
+    a.writeG = stage != 2;
+    // element order alternates with the trace buffer parity, i.e. from stage to stage
+    a.reverse = (kind != SELFAMD_MODEL_SW || !tunable(kind, N) || tile_alternates(N)) ? traceIn : 0;
     a.perElemDt = 0;
     a.p = p;
     switch (kind) {
@@ -1085,12 +1097,12 @@ int selfamd_tile_count(int kind, int N, int *count) {
   return kOk;
 }
 
-int selfamd_tile_info(int kind, int N, int variant, int *threads, int *wavesPerEU, int *xcdOrder) {
-  SELFAMD_REQUIRE(threads && wavesPerEU && xcdOrder, "tile_info: null argument");
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder) {
+  SELFAMD_REQUIRE(threads && alternate && xcdOrder, "tile_info: null argument");
   SELFAMD_REQUIRE(tunable(kind, N) && variant >= 0 && variant < kTileVariants,
                   "tile_info: no variant %d for kind %d, N=%d", variant, kind, N);
   *threads = 64 << (variant >> 2);
-  *wavesPerEU = (variant & 2) ? 5 : 1;
+  *alternate = (variant >> 1) & 1;
   *xcdOrder = variant & 1;
   return kOk;
 }
@@ -1160,6 +1172,8 @@ struct selfamd_batch {
     a.aux = g.aux; a.stat2 = g.stat2; a.presTrace = g.presTrace;
     a.geom = g.geom; a.nbr = g.nbr; a.mats = g.mats; a.nEl = g.nEl;
     a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
+    a.writeG = stage != 2;
+    a.reverse = (kind != SELFAMD_MODEL_SW || !tunable(kind, g.N) || tile_alternates(g.N)) ? tin : 0;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);

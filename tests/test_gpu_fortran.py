@@ -29,7 +29,7 @@ def test_fortran_driver_parity(kind, model, N, n, periodic, bcid, tmp_path):
     dt, steps = c.dt(), 6
     par = tmp_path / "run.nml"
     par.write_text(f"&run model={model}, N={N}, nx={n}, ny={n}, periodic={'.true.' if periodic else '.false.'}, "
-                   f"bcid={bcid}, nSteps={steps}, dt={dt!r}, nodal=.false., prefix='{prefix}' /\n")
+                   f"bcid={bcid}, nSteps={steps}, dt={float(dt)!r}, nodal=.false., prefix='{prefix}' /\n")
     r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     shape = c.sol.shape

@@ -306,3 +306,51 @@ def test_threads_bit_identical():
     oracle.set_threads(1)
     np.testing.assert_array_equal(a, b)
     assert rel_err(a, b).max() == 0.0
+
+
+def test_euler_source_closed_form():
+    """A uniform state has zero flux divergence (periodic mesh), so the tendency is the source
+    term alone.  With gravity potential phi = x/2 + y (grad phi = (1/2, 1), exact for the
+    polynomial) and drag Cd, the reference's Source_CompressibleIdealGas2D
+    (src/models/SELF_CompressibleIdealGas2D.f90:1355-1358) gives
+      (rho u)_t = -rho gx - Cd rho u,  (rho v)_t = -rho gy - Cd rho v,  rho_t = 0,
+      (rho E)_t = -rho u gx - rho u gy   (rho u twice: the reference's expression).
+    rho v = -0.2 separates the last line from -rho u gx - rho v gy."""
+    c = Case("euler", 4, 3, 3, periodic=True, generic=False)
+    nel = c.mesh.nElem
+    x = c.geom.x
+    rho, ru, rv, cd = 1.0, 0.3, -0.2, 0.1
+    sol = np.zeros((nel, 4, 5, 5))
+    sol[:, 0], sol[:, 1], sol[:, 2] = ru, rv, rho
+    sol[:, 3] = 1.0 / 0.4 + 0.5 * (ru * ru + rv * rv) / rho
+    c.fields["phi"] = 0.5 * x[..., 0] + x[..., 1]
+    c.fields["drag"] = np.full((nel, 5, 5), cd)
+    c.sol = sol
+    c.apply_oracle(c.om)
+    t = c.om.tendency()
+    gx, gy = 0.5, 1.0
+    want = [-rho * gx - cd * ru, -rho * gy - cd * rv, 0.0, -ru * gx - ru * gy]
+    for v in range(4):
+        assert np.abs(t[:, v] - want[v]).max() < 1e-12, (v, t[:, v].min(), t[:, v].max(), want[v])
+
+
+def test_sw_velocity_lag_first_order_in_time():
+    """The SW flux uses face normal velocities lagged by one RK stage
+    (src/models/SELF_ShallowWater.f90:222-237: PreTendency updates the velocity traces after
+    they are used).  The stage tendency then carries an O(dt) error, so the RK3 step of the
+    nonlinear system converges at first order in time, not third; a restatement without the
+    lag would converge at third order, like the linear system in test_rk3_third_order."""
+    c = Case("sw", 3, 4, 4, periodic=True, seed=3)
+    c.sol[:, 0] *= 30.0  # O(0.3) velocities: the advective flux matters
+    c.sol[:, 1] *= 30.0
+    T = 0.02
+    sols = []
+    for n in (8, 16, 32, 512):
+        om = oracle.Model(oracle.SW, 3, c.mesh.sideInfo, c.geom)
+        c.apply_oracle(om)
+        om.rk3_steps(n, T / n)
+        sols.append(om.solution)
+    ref = sols[-1]
+    e = [np.abs(s - ref).max() for s in sols[:3]]
+    rates = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    assert np.all((rates > 0.8) & (rates < 1.5)), rates
