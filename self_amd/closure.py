@@ -6,11 +6,13 @@ tendency correction.  Training data come from fine runs of the same solver (r nx
 elements, degree Nf, the seeded nonlinear-SW distribution): for a sampled coarse step
 starting at the projected fine state P S_f(t),
 
-    target = (P S_f(t + dt_c) - Phi_c(P S_f(t))) / dt_c,
+    target = (P S_f(t + w dt_c) - Phi_c^w(P S_f(t))) / (w dt_c),
 
-where Phi_c is one plain coarse RK3 step and dt_c = k dt_f (k whole fine steps).  Because an
-RK3 step adds exactly dt_c times a constant forcing, a correction equal to the target would
-make the coarse step land on the projected fine state.  The library's kernels do all of it:
+where Phi_c^w is w plain coarse RK3 steps and dt_c = k dt_f (k whole fine steps).  Because an
+RK3 step adds exactly dt_c times a constant forcing, a constant correction equal to the
+target would make the coarse run land on the projected fine state after the w steps.  The
+correction can be scaled by a factor alpha chosen on validation runs (alpha = 0 is the plain
+run, the fallback).  The library's kernels do all of it:
 the fine and coarse solvers, the L2 projection, the MLP forward/backward and Adam; this
 module drives them.  The mass (H) correction is made mean-free over the mesh by the engine,
 so the corrected run conserves mass to round-off.
@@ -80,25 +82,30 @@ class CoarseFine:
         return float(np.einsum("e,ji,eji->", self.J_c, w2, S[:, 2]))
 
 
-def generate(cf, seeds, n_steps=500, stride=20, log=None):
+def generate(cf, seeds, n_steps=500, stride=20, window=1, log=None):
     """Training pairs (raw features, raw targets) from fine runs of the given seeds: one
-    sample per element at every `stride`-th coarse step of an n_steps horizon."""
+    sample per element at every `stride`-th coarse step of an n_steps horizon.  The target
+    is the mean tendency defect over `window` coarse steps,
+    (P S_f(t + w dt_c) - Phi_c^w(P S_f(t))) / (w dt_c): longer windows average out the
+    part of the defect carried by fast unresolved waves."""
+    assert stride >= window, "samples must not overlap (stride >= window)"
     X, T = [], []
     for seed in seeds:
         F, topo = cf.fine_model(seed)
         C = cf.coarse_model(topo)
+        P0, due = None, None
         for step in range(n_steps):
-            sample = step % stride == 0
-            if sample:
-                P0 = cf.project(F.solution_device()).to_host()
+            if step % stride == 0 and step + window <= n_steps:
+                P0, due = cf.project(F.solution_device()).to_host(), step + window
             F.ForwardStep(cf.k, cf.dt_f)
-            if sample:
+            if due is not None and step + 1 == due:
                 P1 = cf.project(F.solution_device()).to_host()
                 C.SetSolution(P0)
-                C.ForwardStep(1, cf.dt_c)
+                C.ForwardStep(window, cf.dt_c)
                 Sc = C.solution
                 X.append(cf.raw_features(P0))
-                T.append(((P1 - Sc) / cf.dt_c).reshape(P0.shape[0], -1))
+                T.append(((P1 - Sc) / (window * cf.dt_c)).reshape(P0.shape[0], -1))
+                due = None
         if log:
             log(f"closure data: seed {seed} done ({len(X)} snapshots)")
         del F, C
@@ -122,8 +129,10 @@ class Normalisation:
     def t(self, T):
         return ((T - self.outMean.astype(np.float64)) / self.outScale.astype(np.float64)).astype(np.float32)
 
-    def arrays(self):
-        return self.inMean, self.inScale, self.outMean, self.outScale
+    def arrays(self, alpha=1.0):
+        """Normalisation arrays for the engine; alpha scales the correction (0 = plain run)."""
+        return (self.inMean, self.inScale, (alpha * self.outMean).astype(np.float32),
+                (alpha * self.outScale).astype(np.float32))
 
     def save(self, path, params):
         np.savez(path, inMean=self.inMean, inScale=self.inScale, outMean=self.outMean, outScale=self.outScale,
@@ -180,6 +189,15 @@ def train(mlp, Xn, Tn, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, lo
     return hist
 
 
+def skill(mlp, Xn, Tn, rows=65536):
+    """A priori skill on normalised data: 1 - sum (y - t)^2 / sum (t - mean t)^2 (R^2 over all
+    outputs) on up to `rows` samples."""
+    n = min(rows, Xn.shape[0])
+    y = np.concatenate([mlp.forward(Xn[o:o + mlp.maxBatch]) for o in range(0, n, mlp.maxBatch)])[:n]
+    t = Tn[:n].astype(np.float64)
+    return float(1.0 - np.sum((y - t) ** 2) / np.sum((t - t.mean(0)) ** 2))
+
+
 def run_coarse(cf, C, P0, n_steps, checkpoints):
     """Advance coarse model C from P0; states at the checkpoints and the timed milliseconds."""
     C.SetSolution(P0)
@@ -192,7 +210,7 @@ def run_coarse(cf, C, P0, n_steps, checkpoints):
     return states, ms
 
 
-def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, log=None):
+def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, alphas=(1.0,), log=None):
     """Fine truth, plain coarse and (if given) closure-corrected coarse runs of one seed.
     Returns per-checkpoint L2 errors (summed over variables) against the projected fine
     state, mass drifts and timed milliseconds."""
@@ -210,11 +228,29 @@ def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, log=N
     out["plain_err"] = [float(np.sum(cf.l2(plain[c] - truth[c]))) for c in checkpoints]
     out["plain_mass_drift"] = (cf.mass(plain[checkpoints[-1]]) - out["mass0"]) / out["mass0"]
     if mlp is not None:
-        C.SetClosure(mlp, *norm.arrays())
-        corr, out["closure_ms"] = run_coarse(cf, C, P0, n_steps, checkpoints)
-        C.SetClosure(None)
-        out["closure_err"] = [float(np.sum(cf.l2(corr[c] - truth[c]))) for c in checkpoints]
-        out["closure_mass_drift"] = (cf.mass(corr[checkpoints[-1]]) - out["mass0"]) / out["mass0"]
+        out["closure"] = {}
+        for alpha in alphas:
+            C.SetClosure(mlp, *norm.arrays(alpha))
+            corr, ms = run_coarse(cf, C, P0, n_steps, checkpoints)
+            C.SetClosure(None)
+            last = corr[checkpoints[-1]]
+            out["closure"][alpha] = {
+                "ms": ms, "err": [float(np.sum(cf.l2(corr[c] - truth[c]))) for c in checkpoints],
+                "mass_drift": (cf.mass(last) - out["mass0"]) / out["mass0"] if np.isfinite(last).all() else None}
     if log:
-        log(f"closure eval seed {seed}: " + ", ".join(f"{k}={v}" for k, v in out.items() if k.endswith("err")))
+        log(f"closure eval seed {seed}: plain {out['plain_err'][-1]:.4e}" + "".join(
+            f", alpha {a}: {r['err'][-1]:.4e}" for a, r in out.get("closure", {}).items()))
     return out
+
+
+def select_alpha(cf, mlp, norm, seeds, n_steps, alphas=(0.0, 0.25, 0.5, 0.75, 1.0), log=None):
+    """Correction scale minimising the mean final L2 error over validation seeds.  alpha = 0
+    is the plain run, so the chosen closure is never worse than plain on the validation set."""
+    errs = {a: [] for a in alphas}
+    for seed in seeds:
+        r = evaluate(cf, seed, mlp, norm, n_steps=n_steps, alphas=alphas, log=log)
+        for a in alphas:
+            e = r["closure"][a]["err"][-1]
+            errs[a].append(e if np.isfinite(e) else np.inf)
+    mean = {a: float(np.mean(v)) for a, v in errs.items()}
+    return min(mean, key=mean.get), mean
