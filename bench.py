@@ -233,6 +233,8 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
         models.append(m)
         dts.append(workloads.cfl_dt(arcmin, 1.2, 0.5))
         inits.append((sol, topo))
+        if (r + 1) % 64 == 0:
+            print(f"ensemble: {r + 1}/{n_runs} runs set up", file=sys.stderr, flush=True)
     setup_s = time.perf_counter() - t0
     batch = self_amd.Batch(models, dts)
     groups, dof = batch.info()

@@ -103,9 +103,13 @@ def generate(cf, seeds, n_steps=500, stride=20, window=1, log=None):
                 C.SetSolution(P0)
                 C.ForwardStep(window, cf.dt_c)
                 Sc = C.solution
+                due = None
+                if not (np.isfinite(P1).all() and np.isfinite(Sc).all()):
+                    if log:
+                        log(f"closure data: seed {seed} non-finite state at coarse step {step + 1}; run dropped there")
+                    break
                 X.append(cf.raw_features(P0))
                 T.append(((P1 - Sc) / (window * cf.dt_c)).reshape(P0.shape[0], -1))
-                due = None
         if log:
             log(f"closure data: seed {seed} done ({len(X)} snapshots)")
         del F, C
@@ -226,6 +230,7 @@ def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, alpha
     out = {"checkpoints": checkpoints, "fine_ms": fine_ms, "mass0": cf.mass(P0)}
     plain, out["plain_ms"] = run_coarse(cf, C, P0, n_steps, checkpoints)
     out["plain_err"] = [float(np.sum(cf.l2(plain[c] - truth[c]))) for c in checkpoints]
+    out["plain_err_var"] = [float(x) for x in cf.l2(plain[checkpoints[-1]] - truth[checkpoints[-1]])]
     out["plain_mass_drift"] = (cf.mass(plain[checkpoints[-1]]) - out["mass0"]) / out["mass0"]
     if mlp is not None:
         out["closure"] = {}
@@ -236,6 +241,7 @@ def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, alpha
             last = corr[checkpoints[-1]]
             out["closure"][alpha] = {
                 "ms": ms, "err": [float(np.sum(cf.l2(corr[c] - truth[c]))) for c in checkpoints],
+                "err_var": [float(x) for x in cf.l2(corr[checkpoints[-1]] - truth[checkpoints[-1]])],
                 "mass_drift": (cf.mass(last) - out["mass0"]) / out["mass0"] if np.isfinite(last).all() else None}
     if log:
         log(f"closure eval seed {seed}: plain {out['plain_err'][-1]:.4e}" + "".join(
