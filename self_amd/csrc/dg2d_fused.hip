@@ -59,7 +59,7 @@ struct FusedArgs {
   const double *geom;       // per-element records (affine mode)
   const double *ndsdx, *nJ, *nnHat, *nnScale;  // per-node metrics (nodal mode), SEMQuad shapes
   const int32_t *nbr;       // per (side, element) neighbour code
-  const double *mats;       // dgM(P*P) D(P*P) bM(2P) qW(P)
+  const double *mats;       // dgM(P*P) D(P*P) bM(2P) qW(P) 1/qW(P)
   int nEl;
   double rkA, rkGdt;        // stage coefficients a_m and g_m*dt
   double rkG;               // g_m alone: batched runs take dt from their element records
@@ -82,7 +82,8 @@ struct PhysSW {
   static constexpr bool kAuxGrad = true, kAuxTrace = true, kLag = true;
   __device__ static void volume(const double *s, double, const Params &p, double *fx, double *fy, double *q) {
     const double Hu = s[0], Hv = s[1], H = s[2];
-    const double u = Hu / H, v = Hv / H;  // PreTendency velocity (src/models/SELF_ShallowWater.f90:206-218)
+    const double invH = 1.0 / H;
+    const double u = Hu * invH, v = Hv * invH;  // PreTendency velocity (src/models/SELF_ShallowWater.f90:206-218)
     fx[0] = u * Hu + 0.5 * p.g * H * H;
     fy[0] = v * Hu;
     fx[1] = u * Hv;
@@ -274,7 +275,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   constexpr bool GRADN = NODAL && Phys::kAuxGrad;
   __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NN : 1];
   __shared__ double sBF[NV][EPB][NF];
-  __shared__ double sM[2 * NN + 3 * NP];  // dgM, D, bM(2P), qW
+  __shared__ double sM[2 * NN + 4 * NP];  // dgM, D, bM(2P), qW, 1/qW
 
   const int tid = threadIdx.x;
   const int el = tid / NN, node = tid % NN;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // ---- loads: node lanes ----
   double s[NV], gReg[NV];
   double auxVal = a.p.auxConst, statVal = a.p.stat2Const;
-  double dsdx[4] = {0, 0, 0, 0}, jac = 1.0;
+  double dsdx[4] = {0, 0, 0, 0}, invJ = 1.0;
   if (nodeActive) {
 #pragma unroll
     for (int v = 0; v < NV; v++) s[v] = a.S[sBase + (size_t)v * NN + node];
@@ -308,11 +309,11 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     if constexpr (NODAL) {
       const double *gn = a.ndsdx + 4 * ((size_t)e * NN + node);
       dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
-      jac = a.nJ[(size_t)e * NN + node];
+      invJ = 1.0 / a.nJ[(size_t)e * NN + node];
     } else {
       const double *gr = a.geom + (size_t)e * kGeomStride;
       dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
-      jac = gr[4];
+      invJ = gr[6];  // 1/J, precomputed in the element record
     }
   }
 
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
     }
   }
-  for (int k = tid; k < 2 * NN + 3 * NP; k += THREADS) sM[k] = a.mats[k];
+  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) sM[k] = a.mats[k];
   const double *dgM = sM, *Dm = sM + NN, *bM0 = sM + 2 * NN, *bM1 = bM0 + NP, *qW = bM1 + NP;
 
   // ---- phase 0: volume fluxes, nodal trace quantities ----
@@ -403,7 +404,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       // velocity traces lag one stage (src/models/SELF_ShallowWater.f90:222-237): the stored
       // normal velocities are used now and this stage's are stored for the next one.  The
       // outer side's u.n is the neighbour's own value with the opposite normal.
-      a.unOut[(size_t)fi_ + NP * (side_ + 4 * (size_t)fe_)] = (qL[0] / qL[2]) * nx_ + (qL[1] / qL[2]) * ny_;
+      const double invH = 1.0 / qL[2];
+      a.unOut[(size_t)fi_ + NP * (side_ + 4 * (size_t)fe_)] = (qL[0] * invH) * nx_ + (qL[1] * invH) * ny_;
     }
     double fstar[NV];
     Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, unR_, a.p, fstar);
@@ -445,24 +447,29 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     double gx = 0.0, gy = 0.0;
     if (Phys::kAuxGrad && a.aux) {
       // GradientSF of the static field (src/SELF_MappedData.f90:1080-1149)
-      for (int ii = 0; ii < NP; ii++) {
-        if constexpr (GRADN) {
+      if constexpr (GRADN) {
+        for (int ii = 0; ii < NP; ii++) {
           gx = gx + Dm[ii + NP * i] * sGb[0][el][ii + NP * j] + Dm[ii + NP * j] * sGb[2][el][i + NP * ii];
           gy = gy + Dm[ii + NP * i] * sGb[1][el][ii + NP * j] + Dm[ii + NP * j] * sGb[3][el][i + NP * ii];
-        } else {
-          const double fa = sAux[el][ii + NP * j], fb = sAux[el][i + NP * ii];
-          gx = gx + Dm[ii + NP * i] * (fa * dsdx[0]) + Dm[ii + NP * j] * (fb * dsdx[2]);
-          gy = gy + Dm[ii + NP * i] * (fa * dsdx[1]) + Dm[ii + NP * j] * (fb * dsdx[3]);
         }
+      } else {
+        // affine element: the metric terms are constant, so they factor out of the sums
+        double dxi = 0.0, deta = 0.0;
+        for (int ii = 0; ii < NP; ii++) {
+          dxi = dxi + Dm[ii + NP * i] * sAux[el][ii + NP * j];
+          deta = deta + Dm[ii + NP * j] * sAux[el][i + NP * ii];
+        }
+        gx = dsdx[0] * dxi + dsdx[2] * deta;
+        gy = dsdx[1] * dxi + dsdx[3] * deta;
       }
-      gx = gx / jac;
-      gy = gy / jac;
+      gx = gx * invJ;
+      gy = gy * invJ;
     }
     double src[NV];
     Phys::source(s, gx, gy, statVal, a.p, src);
     // per-run step size of a batched launch: the same IEEE product g_m*dt as a single run
     const double gdt = (MODE == 0 && a.perElemDt) ? a.rkG * a.geom[(size_t)e * kGeomStride + 5] : a.rkGdt;
-    const double wi = qW[i], wj = qW[j];
+    const double iwi = qW[NP + i], iwj = qW[NP + j];  // 1/w_i, 1/w_j
     const double bi1 = bM1[i], bi0 = bM0[i], bj1 = bM1[j], bj0 = bM0[j];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -470,9 +477,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       double d = 0.0;
       for (int ii = 0; ii < NP; ii++)
         d = d + dgM[ii + NP * i] * sF1[v][el][ii + NP * j] + dgM[ii + NP * j] * sF2[v][el][i + NP * ii];
-      d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) / wi +
-          (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) / wj;
-      d = d / jac;
+      d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) * iwi +
+          (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) * iwj;
+      d = d * invJ;
       double dsdt = src[v] - d;
       if (a.corr) {
         // closure: mass correction made mean-free over the mesh (variable 2 is H / eta / rho)
@@ -808,6 +815,7 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
     double scale = 0.0;
     for (int k = 0; k < 4; k++) { r[k] = d0[k]; scale = std::fmax(scale, std::fabs(d0[k])); }
     r[4] = hJ[(size_t)PP * e];
+    r[6] = 1.0 / r[4];
     for (int n = 0; n < PP && affine; n++) {
       for (int k = 0; k < 4; k++)
         if (std::fabs(d0[4 * n + k] - r[k]) > 1e-11 * scale) affine = false;
@@ -828,10 +836,11 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   double cp[8], tp[8], bw[8], qw[8], iM[64], dM[64], dgM[64], bM[16];
   int rc = selfamd_lagrange_init(N, SELFAMD_GAUSS, N, SELFAMD_GAUSS, cp, tp, bw, qw, iM, dM, dgM, bM);
   if (rc) return rc;
-  std::vector<double> mats(2 * PP + 3 * P);
+  std::vector<double> mats(2 * PP + 4 * P);
   for (int k = 0; k < PP; k++) { mats[k] = dgM[k]; mats[PP + k] = dM[k]; }
   for (int k = 0; k < 2 * P; k++) mats[2 * PP + k] = bM[k];
   for (int k = 0; k < P; k++) mats[2 * PP + 2 * P + k] = qw[k];
+  for (int k = 0; k < P; k++) mats[2 * PP + 3 * P + k] = 1.0 / qw[k];
 
   Model *m = new Model();
   m->kind = kind; m->N = N; m->nEl = nEl;
@@ -1241,7 +1250,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
     const size_t nS = total * PP * nV, nTr = total * P * nT * 4;
     if (hipMalloc(&g.S, 8 * nS) || hipMalloc(&g.G, 8 * nS) || hipMalloc(&g.trace[0], 8 * nTr) ||
         hipMalloc(&g.trace[1], 8 * nTr) || hipMalloc(&g.geom, 8 * kGeomStride * total) ||
-        hipMalloc(&g.mats, 8 * (2 * PP + 3 * P)) || hipMalloc(&g.nbr, 4 * 4 * total) ||
+        hipMalloc(&g.mats, 8 * (2 * PP + 4 * P)) || hipMalloc(&g.nbr, 4 * 4 * total) ||
         (m0->un[0] && (hipMalloc(&g.un[0], 8 * total * P * 4) || hipMalloc(&g.un[1], 8 * total * P * 4))) || (m0->aux && hipMalloc(&g.aux, 8 * total * PP)) ||
         (m0->stat2 && hipMalloc(&g.stat2, 8 * total * PP)) || (m0->presTrace && hipMalloc(&g.presTrace, 8 * nTr))) {
       b->groups.push_back(g);
@@ -1249,7 +1258,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
     }
     b->groups.push_back(g);
     BatchGroup &G = b->groups.back();
-    if (hipMemcpy(G.mats, models[G.runs[0]]->mats, 8 * (2 * PP + 3 * P), hipMemcpyDeviceToDevice))
+    if (hipMemcpy(G.mats, models[G.runs[0]]->mats, 8 * (2 * PP + 4 * P), hipMemcpyDeviceToDevice))
       return fail("copy failed");
     std::vector<double> rec;
     std::vector<int32_t> nb;

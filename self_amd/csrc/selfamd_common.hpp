@@ -73,7 +73,7 @@ inline __host__ __device__ size_t scb_idx(int i, int v, int s, int e, int P, int
 }
 
 // Per-element record of an affine element (model engine):
-//   dsdx(4) J(1) dt(1, batched runs) pad(2) nHat_x(4 sides) nHat_y(4) nScale(4)
+//   dsdx(4) J(1) dt(1, batched runs) 1/J(1) pad(1) nHat_x(4 sides) nHat_y(4) nScale(4)
 constexpr int kGeomStride = 20;
 
 inline unsigned grid1d(size_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
