@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed RK3 steps (default per workload)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble"])
+    ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble", "closure"])
+    ap.add_argument("--closure-runs", type=int, default=8, help="fine training runs of the closure workload")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -277,6 +278,34 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
     return 0
 
 
+# ---------------------------------------------------------------------------------------
+# learned closure
+# ---------------------------------------------------------------------------------------
+def run_closure(a, root):
+    from self_amd import closure
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    r = closure.benchmark(n_train=a.closure_runs, n_steps=a.steps or 500, log=log)
+    runs = r["runs"]
+    value = float(np.mean([x["speedup_at_equal_error"] for x in runs]))
+    out = {
+        "metric": "closure speedup at equal L2 error (coarse 64x64 N=3 SW + MLP vs plain runs, fine 256x256 N=7 truth)",
+        "value": value, "unit": "x", "n_gpus": 1, "steps": a.steps or 500, "warmup": 0,
+        "ms_per_step": float(np.mean([x["closure_ms"] for x in runs])) / (a.steps or 500),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "nonlinear SW 2D closure: coarse 64x64 periodic N=3 FP64 + FP32 MLP 51-128-128-48 "
+                               "GELU, trained on fine 256x256 N=7 runs projected onto the coarse space",
+                   "train_runs": a.closure_runs, "train_samples": r["train_samples"], "alpha": r["alpha"],
+                   "skill_train": r["skill_train"], "skill_validation": r["skill_validation"],
+                   "data_s": r["data_s"], "train_s": r["train_s"], "runs": runs},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out, default=float), flush=True)
+    return 0
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -286,6 +315,8 @@ def main():
     root = os.path.dirname(os.path.abspath(__file__))
     if a.workload == "ensemble":
         return run_ensemble(a, root)
+    if a.workload == "closure":
+        return run_closure(a, root)
     return run_single(a, root)
 
 

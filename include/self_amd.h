@@ -295,8 +295,8 @@ int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, do
  * the data the previous one left in the Infinity Cache) and XCD-contiguous element order.
  * No reference counterpart; the autotuner (self_amd/autotune.py) times them and records the
  * fastest per N.  A set variant (-1 = default tile) applies to every model's next graph
- * capture and to batches created afterwards; it never changes results (same arithmetic per
- * element). */
+ * capture and to batches created afterwards.  Variants agree to round-off (blocks with room
+ * for two lanes per face node split the face-line sums between them). */
 int selfamd_tile_count(int kind, int N, int *count);
 int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder);
 int selfamd_tile_set(int kind, int N, int variant);

@@ -3,8 +3,8 @@
 The stage kernel at N = 3, 5, 7 exists in tile variants: work-items per block (64, 128 or
 256), an element order that alternates between stages (each stage starts on the data the
 previous one left in the Infinity Cache) and an XCD-contiguous element order.  `tune` times
-every variant on a periodic SW mesh of the given size, checks that all variants give
-bit-identical states, and returns the fastest.
+every variant on a periodic SW mesh of the given size, checks that the variants agree to
+round-off and returns the fastest.
 `python -m self_amd.autotune` writes the table `self_amd/tuning/sw_tiles.json`, which
 `apply` (called when the package is imported) loads into the library.
 """
@@ -67,8 +67,11 @@ def tune(N, nx, steps=10, warmup=3, repeats=3, log=None):
             s = m.solution
             if ref is None:
                 ref = s
-            elif not np.array_equal(s, ref):
-                raise RuntimeError(f"tile variant {v} at N={N} changed the result")
+            else:
+                # variants agree to round-off (face-line sums may be split across lane pairs)
+                err = np.max(np.abs(s - ref)) / np.max(np.abs(ref))
+                if not err <= 1e-12:
+                    raise RuntimeError(f"tile variant {v} at N={N} changed the result (rel {err:.2e})")
             timings[v] = best / (3 * steps) * 1e3
             if log:
                 log(f"N={N} {nx}x{nx} variant {v:2d} {info(N, v) if v >= 0 else 'default'}: "

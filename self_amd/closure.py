@@ -203,7 +203,10 @@ def skill(mlp, Xn, Tn, rows=65536):
 
 
 def run_coarse(cf, C, P0, n_steps, checkpoints):
-    """Advance coarse model C from P0; states at the checkpoints and the timed milliseconds."""
+    """Advance coarse model C from P0; states at the checkpoints and the timed milliseconds
+    (the step graph is captured by an untimed step first)."""
+    C.SetSolution(P0)
+    C.ForwardStep(1, cf.dt_c)
     C.SetSolution(P0)
     C.t = 0.0
     states, ms, done = {}, 0.0, 0
@@ -260,3 +263,79 @@ def select_alpha(cf, mlp, norm, seeds, n_steps, alphas=(0.0, 0.25, 0.5, 0.75, 1.
             errs[a].append(e if np.isfinite(e) else np.inf)
     mean = {a: float(np.mean(v)) for a, v in errs.items()}
     return min(mean, key=mean.get), mean
+
+
+def plain_ladder(cf, seed, n_steps, factors=(1, 2, 4), log=None):
+    """Plain runs at (q nxc)^2 elements, degree Nc, q in factors, from the projection of the
+    same fine initial state, advanced to the same final time (q times more steps of dt_c / q)
+    and projected onto the coarse mesh.  Returns {q: (L2 error vs the projected fine state,
+    timed ms)} and the fine run's timed ms."""
+    sol_f, topo_f = workloads.nonlinear_sw(cf.xf, cf.interp_f.controlPoints, cf.interp_f.qWeights, seed=seed)
+    F, _ = cf.fine_model(seed)
+    fine_ms = F.time_steps(n_steps * cf.k, cf.dt_f)
+    truth = cf.project(F.solution_device()).to_host()
+    del F
+    sol0 = None
+    out = {}
+    for q in factors:
+        n = cf.nxc * q
+        if cf.nxf % n:
+            continue
+        r = cf.nxf // n
+        mesh = Mesh2D.structured(n, n, periodic=(True, True))
+        geom = SEMQuad(cf.interp_c, mesh)
+        s0 = project_l2_structured(sol_f, cf.Nf, cf.Nc, n, n, r, NVAR).to_host()
+        t0 = project_l2_structured(topo_f.reshape(topo_f.shape[0], 1, cf.Nf + 1, cf.Nf + 1), cf.Nf, cf.Nc, n, n, r,
+                                   1).to_host()
+        m = ShallowWater(mesh, geom)
+        m.SetTopography(t0[:, 0])
+        m.SetSolution(s0)
+        m.ForwardStep(1, cf.dt_c / q)  # capture the step graph outside the timed region
+        m.SetSolution(s0)
+        ms = m.time_steps(n_steps * q, cf.dt_c / q)
+        s = m.solution
+        back = project_l2_structured(s, cf.Nc, cf.Nc, cf.nxc, cf.nxc, q, NVAR).to_host() if q > 1 else s
+        out[q] = (float(np.sum(cf.l2(back - truth))), ms)
+        if log:
+            log(f"plain {n}x{n} N={cf.Nc}: error {out[q][0]:.4e}, {ms:.1f} ms")
+        if q == 1:
+            sol0 = s0
+    return out, fine_ms, sol0
+
+
+def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, seeds_eval=(1000, 1001), log=None):
+    """Closure workload: train on fine runs, pick the correction scale on a validation run,
+    then compare on held-out runs the corrected coarse run with plain runs of increasing
+    resolution.  'speedup_at_equal_error' is the wall-time of the cheapest plain run whose
+    error is at most the corrected run's, divided by the corrected run's wall-time (the fine
+    run itself when no coarser plain run reaches that error)."""
+    import time
+    t0 = time.perf_counter()
+    cf = CoarseFine(nxc=nxc)
+    X, T = generate(cf, range(n_train), n_steps=n_steps, stride=stride, window=window, log=log)
+    Xv, Tv = generate(cf, [500], n_steps=n_steps, stride=stride, window=window)
+    gen_s = time.perf_counter() - t0
+    norm = Normalisation(X, T)
+    mlp = MLP(cf.nIn, 128, cf.nOut, 8192)
+    mlp.set_params(init_params(cf.nIn, 128, cf.nOut, seed=0))
+    t1 = time.perf_counter()
+    hist = train(mlp, norm.x(X), norm.t(T), epochs=epochs, batch=4096, log=log)
+    train_s = time.perf_counter() - t1
+    res = {"train_samples": int(X.shape[0]), "data_s": gen_s, "train_s": train_s, "loss": hist,
+           "skill_train": skill(mlp, norm.x(X), norm.t(T)), "skill_validation": skill(mlp, norm.x(Xv), norm.t(Tv))}
+    alpha, mean = select_alpha(cf, mlp, norm, [600], n_steps, alphas=(0.0, 0.02, 0.05, 0.1, 0.2), log=log)
+    res["alpha"], res["alpha_validation_err"] = alpha, mean
+    runs = []
+    for seed in seeds_eval:
+        r = evaluate(cf, seed, mlp, norm, n_steps=n_steps, alphas=(alpha,), log=log)
+        ladder, fine_ms, _ = plain_ladder(cf, seed, n_steps, log=log)
+        c = r["closure"][alpha]
+        err, ms = c["err"][-1], c["ms"]
+        cheaper = [(v[1], q) for q, v in ladder.items() if v[0] <= err]
+        ref_ms = min(cheaper)[0] if cheaper else fine_ms
+        runs.append({"seed": seed, "closure_err": err, "closure_ms": ms, "plain_err": ladder[1][0],
+                     "plain_ms": ladder[1][1], "ladder": {str(q): v for q, v in ladder.items()}, "fine_ms": fine_ms,
+                     "closure_mass_drift": c["mass_drift"], "speedup_at_equal_error": ref_ms / ms,
+                     "speedup_vs_fine": fine_ms / ms})
+    res["runs"] = runs
+    return res

@@ -266,6 +266,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   constexpr int NV = Phys::NV, NT = Phys::NT;
   constexpr bool USE_AUX = Phys::kAuxGrad || Phys::kAuxTrace;
   constexpr bool ONE_FACE_PASS = EPB * NF <= THREADS;  // every face node has its own lane
+  // two lanes per face node (lanes l and l ^ 32 of a wavefront): each interpolates half of
+  // the face line, the halves are summed across the pair
+  constexpr bool SPLIT = ONE_FACE_PASS && 2 * EPB * NF <= THREADS && NF <= 32;
+  constexpr int FPW = SPLIT ? 32 : 64;  // face-node slots per wavefront
   static_assert(EPB >= 1, "block too small for one element");
   __shared__ double sQ[NT][EPB][NN];
   __shared__ double sF1[NV][EPB][NN];
@@ -318,13 +322,18 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   }
 
   // ---- loads: face lanes (neighbour traces, lagged velocities, normals) ----
-  const int fel = tid / NF, fnode = tid % NF;
+  // SPLIT: wavefront w covers the face nodes of element w (NF <= 32 slots), lanes l and l + 32
+  // share face node l; otherwise face node = thread index
+  const int fhalf = SPLIT ? (tid >> 5) & 1 : 0;
+  const int fslot = SPLIT ? (tid & 31) : 0;
+  const int fidx = SPLIT ? (tid >> 6) * NF + fslot : tid;
+  const int fel = fidx / NF, fnode = fidx % NF;
   const int side = fnode / NP, fi = fnode % NP;
   const int fe = blk * EPB + fel;
-  const bool faceActive = ONE_FACE_PASS && fel < EPB && fe < a.nEl;
+  const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && fe < a.nEl;
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   int code = 0;
-  if (faceActive) {
+  if (faceActive && fhalf == 0) {
     code = a.nbr[4 * (size_t)fe + side];
     if (code >= 0) {
       const int flip = code & 1, s2 = (code >> 1) & 3;
@@ -373,16 +382,38 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     // own traces, ScalarBoundaryInterp_2D order (src/SELF_Lagrange.f90:2241-2270)
     const double *bm = (side_ == 1 || side_ == 2) ? bM1 : bM0;
     const bool alongJ = (side_ == 0 || side_ == 2);
+    // nodes of the face line: base + stride * ii (column i = fi for south / north, row j = fi
+    // for east / west)
+    const int nbase = alongJ ? fi_ : NP * fi_, nstride = alongJ ? NP : 1;
     double qL[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qL[k] = 0.0;
     double auxB = a.p.auxConst;
     if (Phys::kAuxTrace && a.aux) auxB = 0.0;
-    for (int ii = 0; ii < NP; ii++) {
-      const int nd = alongJ ? fi_ + NP * ii : ii + NP * fi_;
+    if constexpr (SPLIT) {
+      constexpr int H0 = NP / 2;
+      const int lo = fhalf ? H0 : 0;
 #pragma unroll
-      for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
-      if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
+      for (int t = 0; t < NP - H0; t++) {
+        const int ii = lo + t;
+        if (fhalf == 0 && t >= H0) break;
+        const int nd = nbase + nstride * ii;
+#pragma unroll
+        for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
+        if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
+      }
+#pragma unroll
+      for (int k = 0; k < NT; k++) qL[k] += __shfl_xor(qL[k], 32);
+      if (Phys::kAuxTrace && a.aux) auxB += __shfl_xor(auxB, 32);
+      if (fhalf) return;  // the pair's first lane does the boundary and Riemann work
+    } else {
+#pragma unroll
+      for (int ii = 0; ii < NP; ii++) {
+        const int nd = nbase + nstride * ii;
+#pragma unroll
+        for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
+        if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
+      }
     }
     const bool interior = code_ >= 0;
     if (!interior) {
@@ -507,6 +538,34 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   __syncthreads();
 
   // ---- phase 3: traces of the updated state ----
+  if constexpr (SPLIT) {
+    if (faceActive) {
+      const double *bm = (side == 1 || side == 2) ? bM1 : bM0;
+      const bool alongJ = (side == 0 || side == 2);
+      const int nbase = alongJ ? fi : NP * fi, nstride = alongJ ? NP : 1;
+      constexpr int H0 = NP / 2;
+      const int lo = fhalf ? H0 : 0;
+      double qn[NT];
+#pragma unroll
+      for (int k = 0; k < NT; k++) qn[k] = 0.0;
+#pragma unroll
+      for (int t = 0; t < NP - H0; t++) {
+        const int ii = lo + t;
+        if (fhalf == 0 && t >= H0) break;
+        const int nd = nbase + nstride * ii;
+#pragma unroll
+        for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel][nd];
+      }
+#pragma unroll
+      for (int k = 0; k < NT; k++) qn[k] += __shfl_xor(qn[k], 32);
+      if (fhalf == 0) {
+        double *tr = a.traceOut + (size_t)fi + NP * (size_t)NT * (side + 4 * (size_t)fe);
+#pragma unroll
+        for (int k = 0; k < NT; k++) tr[(size_t)k * NP] = qn[k];
+      }
+    }
+    return;
+  }
   for (int ft = tid; ft < EPB * NF; ft += THREADS) {
     const int fel3 = ft / NF, fnode3 = ft % NF;
     const int side3 = fnode3 / NP, fi3 = fnode3 % NP;
@@ -514,11 +573,13 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     if (fe3 >= a.nEl) continue;
     const double *bm = (side3 == 1 || side3 == 2) ? bM1 : bM0;
     const bool alongJ = (side3 == 0 || side3 == 2);
+    const int nbase = alongJ ? fi3 : NP * fi3, nstride = alongJ ? NP : 1;
     double qn[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qn[k] = 0.0;
+#pragma unroll
     for (int ii = 0; ii < NP; ii++) {
-      const int nd = alongJ ? fi3 + NP * ii : ii + NP * fi3;
+      const int nd = nbase + nstride * ii;
 #pragma unroll
       for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel3][nd];
     }
@@ -595,7 +656,8 @@ constexpr int tile_threads() {
 // variants, v = 4 t + 2 r + x with THREADS = 64 << t, element order alternating between
 // stages when r = 1, XCD-contiguous element order when x = 1.  The table below (per N; -1 =
 // the default tile) is set by selfamd_tile_set, normally from the autotuner's measurements.
-// Every variant performs the same arithmetic per element (bit-identical results).
+// Variants whose blocks leave room for two lanes per face node split the face-line sums
+// between the pair (round-off level differences); otherwise the arithmetic is identical.
 constexpr int kTileVariants = 12;
 int g_swTile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 // element order alternates between stages (the default tiles alternate too)
@@ -701,6 +763,10 @@ struct selfamd_model2d {
   double t = 0.0;
   // captured RK3 step graphs, one per starting trace buffer
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // kGraphSteps steps per graph (an even number of stage flips: same start parity after),
+  // for long runs of small meshes, where per-graph launch overhead shows
+  static constexpr int kGraphSteps = 8;
+  hipGraphExec_t graphK[2] = {nullptr, nullptr};
   double graphDt = -1.0;
   int graphEpoch = -1;  // tile table version the graphs were captured with
   ClosureBuffers *closure = nullptr;
@@ -745,7 +811,28 @@ struct selfamd_model2d {
   void drop_graphs() {
     for (auto &g : graph)
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &g : graphK)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     graphDt = -1.0;
+  }
+  // captures `steps` RK3 steps starting from trace buffer `start`
+  hipError_t capture(int start, int steps, double dt, hipGraphExec_t *out) {
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    int tin = start;
+    hipError_t le = hipSuccess;
+    for (int n = 0; n < steps; n++)
+      for (int st = 0; st < 3 && le == hipSuccess; st++) {
+        le = launch_stage(0, st, dt, tin, nullptr);
+        tin ^= 1;
+      }
+    hipError_t ce = hipStreamEndCapture(stream, &g);
+    if (le != hipSuccess) return le;
+    if (ce != hipSuccess) return ce;
+    hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    return ie;
   }
 };
 
@@ -995,26 +1082,18 @@ int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
   if (nSteps == 0) return kOk;
   if (m->graphDt != dt || m->graphEpoch != g_tileEpoch) {
     m->drop_graphs();
-    for (int start = 0; start < 2; start++) {
-      hipGraph_t g;
-      SELFAMD_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-      int tin = start;
-      hipError_t le = hipSuccess;
-      for (int st = 0; st < 3 && le == hipSuccess; st++) {
-        le = m->launch_stage(0, st, dt, tin, nullptr);
-        tin ^= 1;
-      }
-      hipError_t ce = hipStreamEndCapture(m->stream, &g);
-      SELFAMD_HIP(le);
-      SELFAMD_HIP(ce);
-      hipError_t ie = hipGraphInstantiate(&m->graph[start], g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      SELFAMD_HIP(ie);
-    }
+    for (int start = 0; start < 2; start++) SELFAMD_HIP(m->capture(start, 1, dt, &m->graph[start]));
     m->graphDt = dt;
     m->graphEpoch = g_tileEpoch;
   }
-  for (int n = 0; n < nSteps; n++) {
+  const int K = selfamd_model2d::kGraphSteps;
+  if (nSteps >= K && !m->graphK[m->cur]) SELFAMD_HIP(m->capture(m->cur, K, dt, &m->graphK[m->cur]));
+  int n = 0;
+  for (; n + K <= nSteps; n += K) {
+    SELFAMD_HIP(hipGraphLaunch(m->graphK[m->cur], m->stream));  // 3K stage flips: parity unchanged
+    for (int q = 0; q < K; q++) m->t += dt;
+  }
+  for (; n < nSteps; n++) {
     SELFAMD_HIP(hipGraphLaunch(m->graph[m->cur], m->stream));
     m->cur ^= 1;  // three stages flip the trace buffer three times
     m->t += dt;
