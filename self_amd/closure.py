@@ -238,7 +238,9 @@ def evaluate(cf, seed, mlp=None, norm=None, n_steps=500, checkpoints=None, alpha
     if mlp is not None:
         out["closure"] = {}
         for alpha in alphas:
-            C.SetClosure(mlp, *norm.arrays(alpha))
+            # alpha = 0 is the fallback: the plain run, without evaluating the network
+            if alpha != 0.0:
+                C.SetClosure(mlp, *norm.arrays(alpha))
             corr, ms = run_coarse(cf, C, P0, n_steps, checkpoints)
             C.SetClosure(None)
             last = corr[checkpoints[-1]]
