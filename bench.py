@@ -81,14 +81,14 @@ def traffic_probe(self_amd, model, dt):
     return 0
 
 
-def tile_name(self_amd, N):
+def tile_name(self_amd, N, phys="PhysSW", kind=2):
     from self_amd import autotune
-    v = autotune.get_tile(N) if autotune.count(N) else -1
+    v = autotune.get_tile(N, kind) if autotune.count(N, kind) else -1
     if v < 0:
         t = 64 if N >= 6 else (128 if N >= 4 else 256)
-        return f"k_fused<PhysSW, {N}, 0, false, {t}, 1, false>"
-    i = autotune.info(N, v)
-    return f"k_fused<PhysSW, {N}, 0, false, {i['threads']}, 1, {str(i['xcd_order']).lower()}>"
+        return f"k_fused<{phys}, {N}, 0, false, {t}, 1, false>"
+    i = autotune.info(N, v, kind)
+    return f"k_fused<{phys}, {N}, 0, false, {i['threads']}, 1, {str(i['xcd_order']).lower()}>"
 
 
 # ---------------------------------------------------------------------------------------
@@ -189,8 +189,8 @@ def run_single(a, root):
     if not a.no_cpu_baseline:
         import oracle
         cpu = cpu_single(oracle, a.workload, w, a.cpu_steps or {"sw256": 100, "euler512": 5, "lsw10": 200}[a.workload])
-    kernel = tile_name(self_amd, N) if a.workload == "sw256" else (
-        f"k_fused<{'PhysEuler' if a.workload == 'euler512' else 'PhysLSW'}, {N}, 0, false, 64, 1, false>")
+    phys, kind = {"sw256": ("PhysSW", 2), "euler512": ("PhysEuler", 3), "lsw10": ("PhysLSW", 1)}[a.workload]
+    kernel = tile_name(self_amd, N, phys, kind)
     traffic = traffic_from_profiles(root, "sw2d_stage_256x256_N7_traffic.json") if a.workload == "sw256" else None
     out = {
         "metric": METRIC if a.workload == "sw256" else f"DOF-updates/sec + % HBM roofline, {a.workload}",

@@ -78,6 +78,7 @@ struct FusedArgs {
 // Nonlinear shallow water (src/models/SELF_ShallowWater.f90): state (Hu, Hv, H), traces
 // are the state traces, aux = topography (gradient in the source, trace in radiation BC).
 struct PhysSW {
+  static constexpr int kKind = SELFAMD_MODEL_SW;
   static constexpr int NV = 3, NT = 3;
   static constexpr bool kAuxGrad = true, kAuxTrace = true, kLag = true;
   __device__ static void volume(const double *s, double, const Params &p, double *fx, double *fy, double *q) {
@@ -129,6 +130,7 @@ struct PhysSW {
 // Linear shallow water (src/models/SELF_LinearShallowWater.f90): state (u, v, eta),
 // aux = bathymetry H (flux and trace for c), stat2 = Coriolis f (source).
 struct PhysLSW {
+  static constexpr int kKind = SELFAMD_MODEL_LSW;
   static constexpr int NV = 3, NT = 3;
   static constexpr bool kAuxGrad = false, kAuxTrace = true, kLag = false;
   __device__ static void volume(const double *s, double H, const Params &p, double *fx, double *fy, double *q) {
@@ -170,21 +172,24 @@ struct PhysLSW {
 // sound speed c (each interpolated from its nodal values, as PreTendency does);
 // aux = gravity potential (gradient in the source), stat2 = drag coefficient.
 struct PhysEuler {
+  static constexpr int kKind = SELFAMD_MODEL_EULER;
   static constexpr int NV = 4, NT = 9;
   static constexpr bool kAuxGrad = true, kAuxTrace = false, kLag = false;
-  __device__ static void nodal_p(const double *s, const Params &p, double &u, double &v, double &pr, double &ke) {
-    u = s[0] / s[2];
-    v = s[1] / s[2];
+  __device__ static void nodal_p(const double *s, const Params &p, double &u, double &v, double &pr, double &ke,
+                                 double &invRho) {
+    invRho = 1.0 / s[2];
+    u = s[0] * invRho;
+    v = s[1] * invRho;
     ke = 0.5 * (s[0] * u + s[1] * v);
     pr = (p.gamma - 1.0) * (s[3] - ke);
   }
   __device__ static void nodal_q(const double *s, const Params &p, double *q) {
-    double u, v, pr, ke;
-    nodal_p(s, p, u, v, pr, ke);
+    double u, v, pr, ke, invRho;
+    nodal_p(s, p, u, v, pr, ke, invRho);
     q[0] = s[0]; q[1] = s[1]; q[2] = s[2]; q[3] = s[3];
     q[4] = u; q[5] = v; q[6] = pr;
-    q[7] = s[3] + pr;                   // diagnostics(2): enthalpy
-    q[8] = sqrt(p.gamma * pr / s[2]);   // diagnostics(3): sound speed
+    q[7] = s[3] + pr;                        // diagnostics(2): enthalpy
+    q[8] = sqrt(p.gamma * pr * invRho);      // diagnostics(3): sound speed
   }
   __device__ static void volume(const double *s, double, const Params &p, double *fx, double *fy, double *q) {
     nodal_q(s, p, q);
@@ -652,19 +657,25 @@ constexpr int tile_threads() {
   return N == 7 ? 64 : N == 6 ? 64 : N == 5 ? 128 : N == 4 ? 128 : 256;
 }
 
-// Tuned tiles: the SW stage kernel (affine metrics, RK update) at N = 3, 5, 7 comes in 12
-// variants, v = 4 t + 2 r + x with THREADS = 64 << t, element order alternating between
+// Tuned tiles: the stage kernel (affine metrics, RK update) of SW at N = 3, 5, 7 and of LSW
+// and Euler at N = 7 comes in 12 variants, v = 4 t + 2 r + x with THREADS = 64 << t, element order alternating between
 // stages when r = 1, XCD-contiguous element order when x = 1.  The table below (per N; -1 =
 // the default tile) is set by selfamd_tile_set, normally from the autotuner's measurements.
 // Variants whose blocks leave room for two lanes per face node split the face-line sums
 // between the pair (round-off level differences); otherwise the arithmetic is identical.
 constexpr int kTileVariants = 12;
-int g_swTile[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+// g_tile[kind][N]: SW at N = 3, 5, 7; LSW and Euler at N = 7
+int g_tile[4][8] = {{-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1},
+                    {-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1}};
+inline bool tunable(int kind, int N) {
+  return (kind == SELFAMD_MODEL_SW && (N == 3 || N == 5 || N == 7)) ||
+         ((kind == SELFAMD_MODEL_LSW || kind == SELFAMD_MODEL_EULER) && N == 7);
+}
+inline int tile_of(int kind, int N) { return tunable(kind, N) ? g_tile[kind][N] : -1; }
 // element order alternates between stages (the default tiles alternate too)
-inline bool tile_alternates(int N) { return g_swTile[N] < 0 || (g_swTile[N] & 2); }
+inline bool tile_alternates(int kind, int N) { const int v = tile_of(kind, N); return v < 0 || (v & 2); }
 int g_tileEpoch = 0;
 
-inline bool tunable(int kind, int N) { return kind == SELFAMD_MODEL_SW && (N == 3 || N == 5 || N == 7); }
 
 template <class Phys, int N>
 hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
@@ -687,12 +698,13 @@ hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
 template <class Phys>
 struct Launcher {
   static hipError_t stage(int N, int mode, bool nodal, const FusedArgs &a, hipStream_t st) {
-    if constexpr (std::is_same<Phys, PhysSW>::value) {
-      if (mode == 0 && !nodal && N >= 1 && N <= 7 && g_swTile[N] >= 0) {
-        if (N == 3) return launch_tile<Phys, 3>(g_swTile[N], a, st);
-        if (N == 5) return launch_tile<Phys, 5>(g_swTile[N], a, st);
-        if (N == 7) return launch_tile<Phys, 7>(g_swTile[N], a, st);
+    const int v = (mode == 0 && !nodal) ? tile_of(Phys::kKind, N) : -1;
+    if (v >= 0) {
+      if constexpr (std::is_same<Phys, PhysSW>::value) {
+        if (N == 3) return launch_tile<Phys, 3>(v, a, st);
+        if (N == 5) return launch_tile<Phys, 5>(v, a, st);
       }
+      if (N == 7) return launch_tile<Phys, 7>(v, a, st);
     }
     switch (N) {
 #define CASE(NN_)                                                                                     \
@@ -792,7 +804,7 @@ struct selfamd_model2d {
     a.rkG = kRk3G[stage];
     a.writeG = stage != 2;
     // element order alternates with the trace buffer parity, i.e. from stage to stage
-    a.reverse = (kind != SELFAMD_MODEL_SW || !tunable(kind, N) || tile_alternates(N)) ? traceIn : 0;
+    a.reverse = tile_alternates(kind, N) ? traceIn : 0;
     a.perElemDt = 0;
     a.p = p;
     switch (kind) {
@@ -1198,14 +1210,14 @@ int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate
 int selfamd_tile_set(int kind, int N, int variant) {
   SELFAMD_REQUIRE(tunable(kind, N), "tile_set: kind %d, N=%d has no tile variants", kind, N);
   SELFAMD_REQUIRE(variant >= -1 && variant < kTileVariants, "tile_set: variant %d out of range", variant);
-  g_swTile[N] = variant;
+  g_tile[kind][N] = variant;
   g_tileEpoch++;
   return kOk;
 }
 
 int selfamd_tile_get(int kind, int N, int *variant) {
   SELFAMD_REQUIRE(variant, "tile_get: null argument");
-  *variant = tunable(kind, N) ? g_swTile[N] : -1;
+  *variant = tile_of(kind, N);
   return kOk;
 }
 
@@ -1261,7 +1273,7 @@ struct selfamd_batch {
     a.geom = g.geom; a.nbr = g.nbr; a.mats = g.mats; a.nEl = g.nEl;
     a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
     a.writeG = stage != 2;
-    a.reverse = (kind != SELFAMD_MODEL_SW || !tunable(kind, g.N) || tile_alternates(g.N)) ? tin : 0;
+    a.reverse = tile_alternates(kind, g.N) ? tin : 0;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);

@@ -79,6 +79,23 @@ def test_tendency_then_steps_keeps_lag_state():
     assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
+@pytest.mark.parametrize("kind", ["sw", "euler"])
+def test_multi_step_graph_matches_single_steps(kind):
+    """Long runs replay an 8-step graph; the state (and model time) must be the same as after
+    the same steps replayed one at a time, and match the oracle."""
+    c = Case(kind, 5, 4, 4, periodic=(kind == "sw"), seed=4, metrics="affine")
+    a, _ = c.gpu_model()
+    b, _ = c.gpu_model()
+    dt = c.dt()
+    a.ForwardStep(19, dt)
+    for _ in range(19):
+        b.ForwardStep(1, dt)
+    assert np.array_equal(a.solution, b.solution)
+    assert a.t == b.t
+    c.om.rk3_steps(19, dt)
+    assert rel_err(a.solution, c.om.solution).max() <= TOL
+
+
 def test_hopr_block_mesh_parity():
     import json
     import os
