@@ -301,6 +301,13 @@ int selfamd_tile_count(int kind, int N, int *count);
 int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder);
 int selfamd_tile_set(int kind, int N, int variant);
 int selfamd_tile_get(int kind, int N, int *variant);
+/* Banded stepping (no reference counterpart): nBands > 0 splits the elements into bands of
+ * whole BFS levels of the element graph and runs each RK3 step as a wavefront over the bands
+ * (stage s of band b after stage s-1 of band b+1), so a band's data are reused from the
+ * Infinity Cache across its three stages.  Results are bit-identical to nBands = 0 (the
+ * default, one launch per stage over all elements).  The number of bands used can be lower
+ * (band limits fall between BFS levels).  Not with a closure attached. */
+int selfamd_model2d_set_bands(selfamd_model2d *model, int nBands);
 /* Model time t */
 int selfamd_model2d_get_time(selfamd_model2d *model, double *t);
 int selfamd_model2d_set_time(selfamd_model2d *model, double t);

@@ -308,6 +308,11 @@ class Model2D:
         call("selfamd_model2d_get_solution", self._h, _ptr(self._buf))
         return self._buf
 
+    def SetBands(self, nBands):
+        """Banded stepping over nBands bands of the element graph (0: off); bit-identical
+        results, better Infinity-Cache reuse across the three RK stages."""
+        call("selfamd_model2d_set_bands", self._h, int(nBands))
+
     def SetClosure(self, mlp, inMean=None, inScale=None, outMean=None, outScale=None):
         """Attach an MLP tendency correction (None detaches); normalisation arrays are host
         float32 of the network's input/output widths."""

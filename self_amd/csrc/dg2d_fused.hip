@@ -20,6 +20,7 @@
 //
 // Geometry: the fast path takes affine elements (metric terms constant per element, true
 // for the structured and block meshes); each element carries one 20-double record.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -66,6 +67,7 @@ struct FusedArgs {
   int perElemDt;            // 1: g_m * dt_e with dt_e in the element record (batched runs)
   int writeG;               // 0 at the last RK3 stage: the next step's first stage (a_1 = 0) never reads G
   int reverse;              // 1: blocks take the elements in reverse order (alternate stages)
+  const int32_t *elems;     // the launch's elements (a band), or null: elements 0 .. nEl-1
   const double *corr;       // learned tendency correction (interior layout), or null
   const double *corrShift;  // global mean of the mass-variable correction (one double)
   Params p;
@@ -295,8 +297,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     blk = x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
   }
   if (a.reverse) blk = (int)gridDim.x - 1 - blk;
-  const int e = blk * EPB + el;
-  const bool nodeActive = el < EPB && e < a.nEl;
+  const int pe = blk * EPB + el;  // position in the launch's element list
+  const bool nodeActive = el < EPB && pe < a.nEl;
+  const int e = a.elems ? (nodeActive ? a.elems[pe] : 0) : pe;
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
 
   // ---- loads: node lanes ----
@@ -334,8 +337,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   const int fidx = SPLIT ? (tid >> 6) * NF + fslot : tid;
   const int fel = fidx / NF, fnode = fidx % NF;
   const int side = fnode / NP, fi = fnode % NP;
-  const int fe = blk * EPB + fel;
-  const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && fe < a.nEl;
+  const int pf = blk * EPB + fel;
+  const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && pf < a.nEl;
+  const int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   int code = 0;
   if (faceActive && fhalf == 0) {
@@ -454,8 +458,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     // N = 1, 2: more face nodes than lanes; load and compute face by face
     for (int ft = tid; ft < EPB * NF; ft += THREADS) {
       const int fel2 = ft / NF, fnode2 = ft % NF, side2 = fnode2 / NP, fi2 = fnode2 % NP;
-      const int fe2 = blk * EPB + fel2;
-      if (fe2 >= a.nEl) continue;
+      const int pf2 = blk * EPB + fel2;
+      if (pf2 >= a.nEl) continue;
+      const int fe2 = a.elems ? a.elems[pf2] : pf2;
       double qR2[NT], unL2 = 0.0, unR2 = 0.0, nx2, ny2, nmag2;
       const int code2 = a.nbr[4 * (size_t)fe2 + side2];
       if (code2 >= 0) {
@@ -510,9 +515,14 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       // VectorDGDivergence_2D (src/SELF_Lagrange.f90:1589-1625) then JacobianWeight
-      double d = 0.0;
-      for (int ii = 0; ii < NP; ii++)
-        d = d + dgM[ii + NP * i] * sF1[v][el][ii + NP * j] + dgM[ii + NP * j] * sF2[v][el][i + NP * ii];
+      // xi and eta sums accumulate separately (two independent FMA chains)
+      double dxs = 0.0, des = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < NP; ii++) {
+        dxs = dxs + dgM[ii + NP * i] * sF1[v][el][ii + NP * j];
+        des = des + dgM[ii + NP * j] * sF2[v][el][i + NP * ii];
+      }
+      double d = dxs + des;
       d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) * iwi +
           (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) * iwj;
       d = d * invJ;
@@ -574,8 +584,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   for (int ft = tid; ft < EPB * NF; ft += THREADS) {
     const int fel3 = ft / NF, fnode3 = ft % NF;
     const int side3 = fnode3 / NP, fi3 = fnode3 % NP;
-    const int fe3 = blk * EPB + fel3;
-    if (fe3 >= a.nEl) continue;
+    const int pf3 = blk * EPB + fel3;
+    if (pf3 >= a.nEl) continue;
+    const int fe3 = a.elems ? a.elems[pf3] : pf3;
     const double *bm = (side3 == 1 || side3 == 2) ? bM1 : bM0;
     const bool alongJ = (side3 == 0 || side3 == 2);
     const int nbase = alongJ ? fi3 : NP * fi3, nstride = alongJ ? NP : 1;
@@ -783,7 +794,18 @@ struct selfamd_model2d {
   int graphEpoch = -1;  // tile table version the graphs were captured with
   ClosureBuffers *closure = nullptr;
 
-  hipError_t launch_stage(int mode, int stage, double dt, int traceIn, double *dSdt) {
+  // Banded stepping: the elements split into bands of whole BFS levels of the element graph
+  // (neighbours differ by at most one band); one RK3 step runs stage s of band b in round
+  // b + s, rounds in order and, within a round, stages in order.  A band's data are then
+  // touched three times within ~3 bands' worth of traffic, i.e. while they are still in the
+  // Infinity Cache.  Dependencies (neighbour traces of the previous stage in the buffer of
+  // that stage's parity, none overwritten before its last reader) hold by this order.
+  int nBands = 0;
+  int32_t *bandElems = nullptr;   // elements sorted by band
+  std::vector<int> bandStart;     // nBands + 1 offsets into bandElems
+  std::vector<int32_t> hnbr;      // host copy of the neighbour codes
+
+  hipError_t launch_stage(int mode, int stage, double dt, int traceIn, double *dSdt, int band = -1) {
     FusedArgs a{};
     if (closure) {
       hipError_t e = closure_evaluate(*closure, S, geom, mats + 2 * (N + 1) * (N + 1) + 2 * (N + 1), N, nVar, 2, nEl,
@@ -798,6 +820,10 @@ struct selfamd_model2d {
     a.unIn = un[traceIn]; a.unOut = un[traceIn ^ 1];
     a.aux = aux; a.stat2 = stat2; a.presTrace = presTrace;
     a.geom = geom; a.nbr = nbr; a.mats = mats; a.nEl = nEl;
+    if (band >= 0) {
+      a.elems = bandElems + bandStart[band];
+      a.nEl = bandStart[band + 1] - bandStart[band];
+    }
     a.ndsdx = ndsdx; a.nJ = nJ; a.nnHat = nnHat; a.nnScale = nnScale;
     a.rkA = kRk3A[stage];
     a.rkGdt = kRk3G[stage] * dt;
@@ -834,11 +860,21 @@ struct selfamd_model2d {
     if (e != hipSuccess) return e;
     int tin = start;
     hipError_t le = hipSuccess;
-    for (int n = 0; n < steps; n++)
-      for (int st = 0; st < 3 && le == hipSuccess; st++) {
-        le = launch_stage(0, st, dt, tin, nullptr);
-        tin ^= 1;
+    for (int n = 0; n < steps; n++) {
+      if (nBands > 0) {
+        for (int r = 0; r < nBands + 2 && le == hipSuccess; r++)
+          for (int st = 0; st < 3 && le == hipSuccess; st++) {
+            const int b = r - st;
+            if (b >= 0 && b < nBands) le = launch_stage(0, st, dt, tin ^ (st & 1), nullptr, b);
+          }
+        tin ^= 1;  // three stages
+      } else {
+        for (int st = 0; st < 3 && le == hipSuccess; st++) {
+          le = launch_stage(0, st, dt, tin, nullptr);
+          tin ^= 1;
+        }
       }
+    }
     hipError_t ce = hipStreamEndCapture(stream, &g);
     if (le != hipSuccess) return le;
     if (ce != hipSuccess) return ce;
@@ -943,6 +979,7 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
 
   Model *m = new Model();
   m->kind = kind; m->N = N; m->nEl = nEl;
+  m->hnbr = nbr;
   m->affine = affine;
   m->nodal = !affine;
   m->nVar = selfamd_model2d_nvar(kind);
@@ -1006,6 +1043,7 @@ int selfamd_model2d_destroy(selfamd_model2d *m) {
   free_closure(m->closure);
   double *bufs[] = {m->S,    m->G,    m->trace[0], m->trace[1], m->un[0], m->un[1], m->geom,  m->mats,
                     m->aux,  m->stat2, m->presTrace, m->ndsdx, m->nJ,    m->nnHat, m->nnScale};
+  if (m->bandElems) (void)hipFree(m->bandElems);
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (m->nbr) (void)hipFree(m->nbr);
@@ -1147,6 +1185,7 @@ int selfamd_model2d_set_closure(selfamd_model2d *m, selfamd_mlp *mlp, const floa
   if (!mlp) return kOk;
   SELFAMD_REQUIRE(inMean && inScale && outMean && outScale, "model2d_set_closure: null normalisation array");
   SELFAMD_REQUIRE(m->affine, "model2d_set_closure: the closure reads the affine element records; mesh is not affine");
+  SELFAMD_REQUIRE(m->nBands == 0, "model2d_set_closure: banded stepping is on (selfamd_model2d_set_bands(model, 0))");
   const int P = m->N + 1, NN = P * P;
   int nIn = 0, nOut = 0, maxB = 0;
   mlp_shape(mlp, &nIn, &nOut, &maxB);
@@ -1179,6 +1218,82 @@ int selfamd_model2d_set_closure(selfamd_model2d *m, selfamd_mlp *mlp, const floa
     return kHipError;
   }
   m->closure = c;
+  return kOk;
+}
+
+int selfamd_model2d_set_bands(selfamd_model2d *m, int nBands) {
+  SELFAMD_REQUIRE(m && nBands >= 0, "model2d_set_bands: bad arguments");
+  SELFAMD_REQUIRE(nBands == 0 || !m->closure, "model2d_set_bands: banded stepping does not take a closure");
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  m->drop_graphs();
+  if (m->bandElems) (void)hipFree(m->bandElems);
+  m->bandElems = nullptr;
+  m->bandStart.clear();
+  m->nBands = 0;
+  if (nBands == 0) return kOk;
+  const int nEl = m->nEl;
+  SELFAMD_REQUIRE(nBands >= 2 && nBands <= nEl, "model2d_set_bands: need 2 <= nBands <= nEl (got %d)", nBands);
+  // BFS levels of the element graph (interior neighbours), component after component
+  std::vector<int> level(nEl, -1), queue;
+  queue.reserve(nEl);
+  int base = 0;
+  for (int seed = 0; seed < nEl; seed++) {
+    if (level[seed] >= 0) continue;
+    size_t head = queue.size();
+    level[seed] = base;
+    queue.push_back(seed);
+    int top = base;
+    while (head < queue.size()) {
+      const int e = queue[head++];
+      for (int sd = 0; sd < 4; sd++) {
+        const int c = m->hnbr[4 * (size_t)e + sd];
+        if (c < 0) continue;
+        const int e2 = c >> 3;
+        if (level[e2] < 0) {
+          level[e2] = level[e] + 1;
+          top = std::max(top, level[e2]);
+          queue.push_back(e2);
+        }
+      }
+    }
+    base = top + 1;
+  }
+  const int nLevels = base;
+  std::vector<int> perLevel(nLevels, 0);
+  for (int e = 0; e < nEl; e++) perLevel[level[e]]++;
+  // consecutive whole levels per band, about nEl / nBands elements each
+  std::vector<int> bandOf(nLevels, 0);
+  int b = 0;
+  long acc = 0;
+  for (int l = 0; l < nLevels; l++) {
+    bandOf[l] = b;
+    acc += perLevel[l];
+    if (b < nBands - 1 && acc >= (long)nEl * (b + 1) / nBands) b++;
+  }
+  const int used = bandOf[nLevels - 1] + 1;
+  SELFAMD_REQUIRE(used >= 2, "model2d_set_bands: the element graph has too few BFS levels (%d) for %d bands",
+                  nLevels, nBands);
+  nBands = used;  // a band boundary can only fall between levels
+  for (int e = 0; e < nEl; e++)
+    for (int sd = 0; sd < 4; sd++) {
+      const int c = m->hnbr[4 * (size_t)e + sd];
+      if (c < 0) continue;
+      const int d = bandOf[level[e]] - bandOf[level[c >> 3]];
+      SELFAMD_REQUIRE(d >= -1 && d <= 1, "model2d_set_bands: neighbours %d and %d are more than one band apart", e,
+                      c >> 3);
+    }
+  std::vector<int32_t> order;
+  order.reserve(nEl);
+  std::vector<int> start(nBands + 1, 0);
+  for (int e = 0; e < nEl; e++) start[bandOf[level[e]] + 1]++;
+  for (int k = 0; k < nBands; k++) start[k + 1] += start[k];
+  order.assign(nEl, 0);
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (int e = 0; e < nEl; e++) order[fill[bandOf[level[e]]]++] = e;  // by band, then element index
+  SELFAMD_HIP(hipMalloc(&m->bandElems, sizeof(int32_t) * nEl));
+  SELFAMD_HIP(hipMemcpy(m->bandElems, order.data(), sizeof(int32_t) * nEl, hipMemcpyHostToDevice));
+  m->bandStart = start;
+  m->nBands = nBands;
   return kOk;
 }
 
