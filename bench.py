@@ -88,7 +88,7 @@ def tile_name(self_amd, N, phys="PhysSW", kind=2):
         t = 64 if N >= 6 else (128 if N >= 4 else 256)
         return f"k_fused<{phys}, {N}, 0, false, {t}, 1, false>"
     i = autotune.info(N, v, kind)
-    return f"k_fused<{phys}, {N}, 0, false, {i['threads']}, 1, {str(i['xcd_order']).lower()}>"
+    return f"k_fused<{phys}, {N}, 0, false, {i['threads']}, {i['min_waves']}, {str(i['xcd_order']).lower()}>"
 
 
 # ---------------------------------------------------------------------------------------

@@ -290,15 +290,17 @@ int selfamd_model2d_synchronize(selfamd_model2d *model);
 /* forward_step bracketed by two hipEvents on the model's stream; returns the elapsed
  * milliseconds between them (synchronises). */
 int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, double *elapsedMs);
-/* Tile variants of the nonlinear-SW stage kernel (affine metrics) at N = 3, 5, 7: block size
- * (64, 128, 256 work-items), element order alternating between stages (so a stage starts on
- * the data the previous one left in the Infinity Cache) and XCD-contiguous element order.
+/* Tile variants of the stage kernel (affine metrics) of nonlinear SW at N = 3, 5, 7 and of
+ * LSW and Euler at N = 7: block size (64, 128, 256 work-items), element order alternating
+ * between stages (so a stage starts on the data the previous one left in the Infinity Cache),
+ * XCD-contiguous element order, and an occupancy target (minWaves wavefronts per SIMD; 1 =
+ * none) for the register allocator.
  * No reference counterpart; the autotuner (self_amd/autotune.py) times them and records the
  * fastest per N.  A set variant (-1 = default tile) applies to every model's next graph
  * capture and to batches created afterwards.  Variants agree to round-off (blocks with room
  * for two lanes per face node split the face-line sums between them). */
 int selfamd_tile_count(int kind, int N, int *count);
-int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder);
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder, int *minWaves);
 int selfamd_tile_set(int kind, int N, int variant);
 int selfamd_tile_get(int kind, int N, int *variant);
 /* Banded stepping (no reference counterpart): nBands > 0 splits the elements into bands of

@@ -102,7 +102,7 @@ SIGNATURES = {
     "selfamd_model2d_closure_features": [c_void_p, c_void_p],
     "selfamd_model2d_set_bands": [c_void_p, c_int],
     "selfamd_tile_count": [c_int, c_int, c_void_p],
-    "selfamd_tile_info": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "selfamd_tile_info": [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "selfamd_tile_set": [c_int, c_int, c_int],
     "selfamd_tile_get": [c_int, c_int, c_void_p],
 }

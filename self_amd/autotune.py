@@ -3,8 +3,9 @@
 The stage kernel (affine metrics, RK update) of nonlinear SW at N = 3, 5, 7 and of linear SW
 and Euler at N = 7 exists in tile variants: work-items per block (64, 128 or 256), an element
 order that alternates between stages (each stage starts on the data the previous one left in
-the Infinity Cache) and an XCD-contiguous element order.  `tune` times every variant on a
-mesh of the given size, checks that the variants agree to round-off and returns the fastest.
+the Infinity Cache), an XCD-contiguous element order and an occupancy target for the
+register allocator.  `tune` times every variant on a mesh of the given size, checks that the
+variants agree to round-off and returns the fastest.
 `python -m self_amd.autotune` writes the table `self_amd/tuning/sw_tiles.json`, which `apply`
 (called when the package is imported) loads into the library.
 """
@@ -30,9 +31,9 @@ def count(N, kind=MODEL_SW):
 
 
 def info(N, v, kind=MODEL_SW):
-    t, w, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    call("selfamd_tile_info", kind, N, v, ctypes.byref(t), ctypes.byref(w), ctypes.byref(x))
-    return {"threads": t.value, "alternate": bool(w.value), "xcd_order": bool(x.value)}
+    t, w, x, o = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("selfamd_tile_info", kind, N, v, ctypes.byref(t), ctypes.byref(w), ctypes.byref(x), ctypes.byref(o))
+    return {"threads": t.value, "alternate": bool(w.value), "xcd_order": bool(x.value), "min_waves": o.value}
 
 
 def set_tile(N, v, kind=MODEL_SW):

@@ -299,7 +299,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   if (a.reverse) blk = (int)gridDim.x - 1 - blk;
   const int pe = blk * EPB + el;  // position in the launch's element list
   const bool nodeActive = el < EPB && pe < a.nEl;
-  const int e = a.elems ? (nodeActive ? a.elems[pe] : 0) : pe;
+  int e = a.elems ? (nodeActive ? a.elems[pe] : 0) : pe;
+  // one element per wavefront: its index (and every address derived from it) is uniform, so
+  // the element record and base addresses can live in scalar registers
+  if constexpr (NN == 64) e = __builtin_amdgcn_readfirstlane(e);
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
 
   // ---- loads: node lanes ----
@@ -309,13 +312,6 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   if (nodeActive) {
 #pragma unroll
     for (int v = 0; v < NV; v++) s[v] = a.S[sBase + (size_t)v * NN + node];
-    if (MODE == 0 && a.rkA != 0.0) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = a.G[sBase + (size_t)v * NN + node];
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
-    }
     if (USE_AUX && a.aux) auxVal = a.aux[(size_t)e * NN + node];
     if (a.stat2) statVal = a.stat2[(size_t)e * NN + node];
     if constexpr (NODAL) {
@@ -339,7 +335,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   const int side = fnode / NP, fi = fnode % NP;
   const int pf = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && pf < a.nEl;
-  const int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
+  int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
+  if constexpr (SPLIT && NF == 32 && NN == 64) fe = __builtin_amdgcn_readfirstlane(fe);  // the wavefront's element
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   int code = 0;
   if (faceActive && fhalf == 0) {
@@ -506,6 +503,14 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       gx = gx * invJ;
       gy = gy * invJ;
     }
+    // the RK register is loaded here, not with the state: it is needed only for the update
+    if (MODE == 0 && a.rkA != 0.0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = a.G[sBase + (size_t)v * NN + node];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
+    }
     double src[NV];
     Phys::source(s, gx, gy, statVal, a.p, src);
     // per-run step size of a batched launch: the same IEEE product g_m*dt as a single run
@@ -669,12 +674,14 @@ constexpr int tile_threads() {
 }
 
 // Tuned tiles: the stage kernel (affine metrics, RK update) of SW at N = 3, 5, 7 and of LSW
-// and Euler at N = 7 comes in 12 variants, v = 4 t + 2 r + x with THREADS = 64 << t, element order alternating between
-// stages when r = 1, XCD-contiguous element order when x = 1.  The table below (per N; -1 =
+// and Euler at N = 7 comes in 18 variants.  v < 12: v = 4 t + 2 r + x with THREADS = 64 << t,
+// element order alternating between stages when r = 1, XCD-contiguous element order when
+// x = 1.  v = 12 + 2 t + w: THREADS = 64 << t, alternating order, and an occupancy target of
+// 5 + w wavefronts per SIMD for the register allocator (a few registers spill to scratch).  The table below (per N; -1 =
 // the default tile) is set by selfamd_tile_set, normally from the autotuner's measurements.
 // Variants whose blocks leave room for two lanes per face node split the face-line sums
 // between the pair (round-off level differences); otherwise the arithmetic is identical.
-constexpr int kTileVariants = 12;
+constexpr int kTileVariants = 18;
 // g_tile[kind][N]: SW at N = 3, 5, 7; LSW and Euler at N = 7
 int g_tile[4][8] = {{-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1},
                     {-1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1}};
@@ -684,7 +691,10 @@ inline bool tunable(int kind, int N) {
 }
 inline int tile_of(int kind, int N) { return tunable(kind, N) ? g_tile[kind][N] : -1; }
 // element order alternates between stages (the default tiles alternate too)
-inline bool tile_alternates(int kind, int N) { const int v = tile_of(kind, N); return v < 0 || (v & 2); }
+inline bool tile_alternates(int kind, int N) {
+  const int v = tile_of(kind, N);
+  return v < 0 || v >= 12 || (v & 2);
+}
 int g_tileEpoch = 0;
 
 
@@ -701,6 +711,8 @@ hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
     V(0, 64, 1, false) V(1, 64, 1, true) V(2, 64, 1, false) V(3, 64, 1, true)
     V(4, 128, 1, false) V(5, 128, 1, true) V(6, 128, 1, false) V(7, 128, 1, true)
     V(8, 256, 1, false) V(9, 256, 1, true) V(10, 256, 1, false) V(11, 256, 1, true)
+    V(12, 64, 5, false) V(13, 64, 6, false) V(14, 128, 5, false) V(15, 128, 6, false)
+    V(16, 256, 5, false) V(17, 256, 6, false)
 #undef V
     default: return hipErrorInvalidValue;
   }
@@ -1312,13 +1324,21 @@ int selfamd_tile_count(int kind, int N, int *count) {
   return kOk;
 }
 
-int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder) {
-  SELFAMD_REQUIRE(threads && alternate && xcdOrder, "tile_info: null argument");
+int selfamd_tile_info(int kind, int N, int variant, int *threads, int *alternate, int *xcdOrder, int *minWaves) {
+  SELFAMD_REQUIRE(threads && alternate && xcdOrder && minWaves, "tile_info: null argument");
   SELFAMD_REQUIRE(tunable(kind, N) && variant >= 0 && variant < kTileVariants,
                   "tile_info: no variant %d for kind %d, N=%d", variant, kind, N);
-  *threads = 64 << (variant >> 2);
-  *alternate = (variant >> 1) & 1;
-  *xcdOrder = variant & 1;
+  if (variant < 12) {
+    *threads = 64 << (variant >> 2);
+    *alternate = (variant >> 1) & 1;
+    *xcdOrder = variant & 1;
+    *minWaves = 1;
+  } else {
+    *threads = 64 << ((variant - 12) >> 1);
+    *alternate = 1;
+    *xcdOrder = 0;
+    *minWaves = 5 + ((variant - 12) & 1);
+  }
   return kOk;
 }
 

@@ -18,7 +18,7 @@ def test_tile_table_roundtrip(tmp_path):
     from self_amd import autotune
     before = {(k, N): autotune.get_tile(N, k) for k in (1, 2, 3) for N in (3, 5, 7)}
     try:
-        assert autotune.count(7, 2) == 12 and autotune.count(5, 3) == 0
+        assert autotune.count(7, 2) == 18 and autotune.count(5, 3) == 0
         table = {"entries": {"sw:7": {"variant": 5}, "5": {"variant": 9}, "euler:7": {"variant": 2}}}
         p = tmp_path / "tiles.json"
         p.write_text(json.dumps(table))
@@ -26,7 +26,8 @@ def test_tile_table_roundtrip(tmp_path):
         assert autotune.get_tile(7, 2) == 5
         assert autotune.get_tile(5, 2) == 9
         assert autotune.get_tile(7, 3) == 2
-        assert autotune.info(7, 5) == {"threads": 128, "alternate": False, "xcd_order": True}
+        assert autotune.info(7, 5) == {"threads": 128, "alternate": False, "xcd_order": True, "min_waves": 1}
+        assert autotune.info(7, 15) == {"threads": 128, "alternate": True, "xcd_order": False, "min_waves": 6}
     finally:
         for (k, N), v in before.items():
             if autotune.count(N, k):
