@@ -40,7 +40,9 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed RK3 steps (default per workload)")
-    ap.add_argument("--warmup", type=int, default=10)
+    # untimed steps before the timed region: the GPU reaches its steady clock after about 100
+    # RK3 steps of the SW workload (the first tens of steps run ~10 % slower)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble", "closure"])
     ap.add_argument("--closure-runs", type=int, default=8, help="fine training runs of the closure workload")
     ap.add_argument("--seed", type=int, default=0)
@@ -79,6 +81,25 @@ def traffic_probe(self_amd, model, dt):
     self_amd.synchronize()
     print("probe: done", flush=True)
     return 0
+
+
+def copy_bandwidth(self_amd, nbytes=1 << 30, reps=20):
+    """Achievable streaming rate on this GPU: read + write bytes per second of the library's
+    copy kernel over `nbytes` (past the 256 MB Infinity Cache), in GB/s."""
+    from self_amd._lib import call
+    n = nbytes // 8
+    a = self_amd.DeviceArray((n,), zero=True)
+    b = self_amd.DeviceArray((n,), zero=True)
+    for _ in range(3):
+        call("selfamd_copy_f64", a.ptr, b.ptr, n, None)
+    self_amd.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call("selfamd_copy_f64", a.ptr, b.ptr, n, None)
+    self_amd.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    del a, b
+    return 2 * nbytes / sec / 1e9
 
 
 def tile_name(self_amd, N, phys="PhysSW", kind=2):
@@ -192,6 +213,7 @@ def run_single(a, root):
     phys, kind = {"sw256": ("PhysSW", 2), "euler512": ("PhysEuler", 3), "lsw10": ("PhysLSW", 1)}[a.workload]
     kernel = tile_name(self_amd, N, phys, kind)
     traffic = traffic_from_profiles(root, "sw2d_stage_256x256_N7_traffic.json") if a.workload == "sw256" else None
+    copy_gbs = copy_bandwidth(self_amd)
     out = {
         "metric": METRIC if a.workload == "sw256" else f"DOF-updates/sec + % HBM roofline, {a.workload}",
         "value": dof * steps / sec, "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
@@ -200,7 +222,8 @@ def run_single(a, root):
         "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
-                     "avg_launch_us": launch_s * 1e6, "bytes_per_launch_alg": bpd / 3.0 * dof},
+                     "avg_launch_us": launch_s * 1e6, "bytes_per_launch_alg": bpd / 3.0 * dof,
+                     "copy_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

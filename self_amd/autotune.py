@@ -94,6 +94,11 @@ def tune(N, nx, kind=MODEL_SW, steps=10, warmup=3, repeats=3, log=None):
     make, dt = _model(kind, N, nx)
     before = get_tile(N, kind)
     timings, ref = {}, None
+    # bring the GPU to its steady clock first (the first ~100 steps run slower), so that the
+    # variants timed first are not penalised
+    m = make()
+    m.ForwardStep(100, dt)
+    del m
     try:
         for v in [-1] + list(range(count(N, kind))):
             set_tile(N, v, kind)

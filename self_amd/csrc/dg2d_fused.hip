@@ -245,6 +245,17 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
   else Phys::nodal(s, q);
 }
 
+// Sum of x over the lane pair l, l ^ 32 of a wavefront, returned to both lanes.  The
+// exchange is v_permlane32_swap (gfx950), a VALU operation: it does not use the LDS pipe that
+// a shuffle through ds_bpermute would occupy.  Both lanes get the same sum x_l + x_(l+32).
+__device__ inline double pair_sum32(double x) {
+  const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+  const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  // a[0], b[0]: the value of lane l mod 32; a[1], b[1]: the value of lane l mod 32 + 32
+  return __hiloint2double((int)b[0], (int)a[0]) + __hiloint2double((int)b[1], (int)a[1]);
+}
+
 // ------------------------------------------------------------------------------
 // The fused kernel
 // ------------------------------------------------------------------------------
@@ -278,19 +289,29 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   constexpr bool SPLIT = ONE_FACE_PASS && 2 * EPB * NF <= THREADS && NF <= 32;
   constexpr int FPW = SPLIT ? 32 : 64;  // face-node slots per wavefront
   static_assert(EPB >= 1, "block too small for one element");
-  __shared__ double sQ[NT][EPB][NN];
-  __shared__ double sF1[NV][EPB][NN];
-  __shared__ double sF2[NV][EPB][NN];
-  __shared__ double sAux[USE_AUX ? EPB : 1][USE_AUX ? NN : 1];
+  // LDS layouts.  Node arrays hold node (i, j) at slot(i, j).  With 8 x 8 nodes a row is 64
+  // bytes, and a column access (fixed i, one row per lane group) would find rows 0, 2, 4, 6 in
+  // the same banks; permuting the columns of row j by i ^ (j & 6) spreads them over the banks,
+  // keeps the pairs (2m, 2m + 1) adjacent for 16-byte reads and keeps every row in place, so
+  // row-wise writes stay conflict-free.  Matrices hold entry (c, r) at c + MRS r, with two pad
+  // doubles per 8-double row for the same reason (they are written once per block).
+  constexpr bool SWZ = NP == 8;
+  constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NN;
+  auto slot = [](int i_, int j_) { return SWZ ? NP * j_ + (i_ ^ (j_ & 6)) : i_ + NP * j_; };
+  __shared__ double sQ[NT][EPB][NNS];
+  __shared__ double sF1[NV][EPB][NNS];
+  __shared__ double sF2[NV][EPB][NNS];
+  __shared__ double sAux[USE_AUX ? EPB : 1][USE_AUX ? NNS : 1];
   // nodal mode: static field times the node's contravariant metric, for GradientSF
   constexpr bool GRADN = NODAL && Phys::kAuxGrad;
-  __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NN : 1];
+  __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NNS : 1];
   __shared__ double sBF[NV][EPB][NF];
-  __shared__ double sM[2 * NN + 4 * NP];  // dgM, D, bM(2P), qW, 1/qW
+  __shared__ double sM[2 * NP * MRS + 4 * NP];  // dgM, D (rows of MRS), bM(2P), qW, 1/qW
 
   const int tid = threadIdx.x;
   const int el = tid / NN, node = tid % NN;
   const int i = node % NP, j = node / NP;
+  const int nl = slot(i, j);  // the node's LDS slot
   int blk = blockIdx.x;
   if constexpr (XCD) {
     const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, x = blk & 7, k = blk >> 3;
@@ -312,6 +333,13 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   if (nodeActive) {
 #pragma unroll
     for (int v = 0; v < NV; v++) s[v] = a.S[sBase + (size_t)v * NN + node];
+    if (MODE == 0 && a.rkA != 0.0) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = a.G[sBase + (size_t)v * NN + node];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
+    }
     if (USE_AUX && a.aux) auxVal = a.aux[(size_t)e * NN + node];
     if (a.stat2) statVal = a.stat2[(size_t)e * NN + node];
     if constexpr (NODAL) {
@@ -331,12 +359,26 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   const int fhalf = SPLIT ? (tid >> 5) & 1 : 0;
   const int fslot = SPLIT ? (tid & 31) : 0;
   const int fidx = SPLIT ? (tid >> 6) * NF + fslot : tid;
-  const int fel = fidx / NF, fnode = fidx % NF;
-  const int side = fnode / NP, fi = fnode % NP;
+  const int fel = fidx / NF, fslotN = fidx % NF;
+  // SPLIT: slots take the sides in the order south, north, east, west, so that the lanes of a
+  // 16-lane LDS group read opposite faces, whose face lines are the same nodes (no bank
+  // conflicts between them)
+  const int side = SPLIT ? ((fslotN / NP) >> 1 | ((fslotN / NP) & 1) << 1) : fslotN / NP;
+  const int fi = fslotN % NP, fnode = side * NP + fi;
   const int pf = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && pf < a.nEl;
   int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
   if constexpr (SPLIT && NF == 32 && NN == 64) fe = __builtin_amdgcn_readfirstlane(fe);  // the wavefront's element
+  // SPLIT: LDS slots of the half face line this lane interpolates (phases 1 and 3).  South and
+  // north faces run along j through column fi, east and west along i through row fi.
+  constexpr int FH0 = NP / 2, FHN = NP - FH0;
+  int foff[SPLIT ? FHN : 1];
+  if constexpr (SPLIT) {
+    const bool alongJ = (side == 0 || side == 2);
+    const int lo = fhalf ? FH0 : 0;
+#pragma unroll
+    for (int t = 0; t < FHN; t++) foff[t] = alongJ ? slot(fi, lo + t) : slot(lo + t, fi);
+  }
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   int code = 0;
   if (faceActive && fhalf == 0) {
@@ -359,8 +401,11 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
     }
   }
-  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) sM[k] = a.mats[k];
-  const double *dgM = sM, *Dm = sM + NN, *bM0 = sM + 2 * NN, *bM1 = bM0 + NP, *qW = bM1 + NP;
+  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) {
+    const int r = k % NN;
+    sM[k < 2 * NN ? (k / NN) * NP * MRS + (r / NP) * MRS + r % NP : 2 * NP * MRS + k - 2 * NN] = a.mats[k];
+  }
+  const double *dgM = sM, *Dm = sM + NP * MRS, *bM0 = sM + 2 * NP * MRS, *bM1 = bM0 + NP, *qW = bM1 + NP;
 
   // ---- phase 0: volume fluxes, nodal trace quantities ----
   if (nodeActive) {
@@ -369,15 +414,15 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       // ContravariantProjection (src/SELF_MappedData.f90:2220-2269)
-      sF1[v][el][node] = dsdx[0] * fx[v] + dsdx[1] * fy[v];
-      sF2[v][el][node] = dsdx[2] * fx[v] + dsdx[3] * fy[v];
+      sF1[v][el][nl] = dsdx[0] * fx[v] + dsdx[1] * fy[v];
+      sF2[v][el][nl] = dsdx[2] * fx[v] + dsdx[3] * fy[v];
     }
 #pragma unroll
-    for (int k = 0; k < NT; k++) sQ[k][el][node] = q[k];
-    if (USE_AUX) sAux[el][node] = auxVal;
+    for (int k = 0; k < NT; k++) sQ[k][el][nl] = q[k];
+    if (USE_AUX) sAux[el][nl] = auxVal;
     if constexpr (GRADN) {
 #pragma unroll
-      for (int k = 0; k < 4; k++) sGb[k][el][node] = auxVal * dsdx[k];
+      for (int k = 0; k < 4; k++) sGb[k][el][nl] = auxVal * dsdx[k];
     }
   }
   __syncthreads();
@@ -387,35 +432,32 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
                   double nx_, double ny_, double nmag_) {
     // own traces, ScalarBoundaryInterp_2D order (src/SELF_Lagrange.f90:2241-2270)
     const double *bm = (side_ == 1 || side_ == 2) ? bM1 : bM0;
+    // nodes of the face line: column i = fi for south / north, row j = fi for east / west
     const bool alongJ = (side_ == 0 || side_ == 2);
-    // nodes of the face line: base + stride * ii (column i = fi for south / north, row j = fi
-    // for east / west)
-    const int nbase = alongJ ? fi_ : NP * fi_, nstride = alongJ ? NP : 1;
     double qL[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qL[k] = 0.0;
     double auxB = a.p.auxConst;
     if (Phys::kAuxTrace && a.aux) auxB = 0.0;
     if constexpr (SPLIT) {
-      constexpr int H0 = NP / 2;
-      const int lo = fhalf ? H0 : 0;
+      const int lo = fhalf ? FH0 : 0;
 #pragma unroll
-      for (int t = 0; t < NP - H0; t++) {
+      for (int t = 0; t < FHN; t++) {
         const int ii = lo + t;
-        if (fhalf == 0 && t >= H0) break;
-        const int nd = nbase + nstride * ii;
+        if (fhalf == 0 && t >= FH0) break;
+        const int nd = foff[t];
 #pragma unroll
         for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
         if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
       }
 #pragma unroll
-      for (int k = 0; k < NT; k++) qL[k] += __shfl_xor(qL[k], 32);
-      if (Phys::kAuxTrace && a.aux) auxB += __shfl_xor(auxB, 32);
+      for (int k = 0; k < NT; k++) qL[k] = pair_sum32(qL[k]);
+      if (Phys::kAuxTrace && a.aux) auxB = pair_sum32(auxB);
       if (fhalf) return;  // the pair's first lane does the boundary and Riemann work
     } else {
 #pragma unroll
       for (int ii = 0; ii < NP; ii++) {
-        const int nd = nbase + nstride * ii;
+        const int nd = alongJ ? slot(fi_, ii) : slot(ii, fi_);
 #pragma unroll
         for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
         if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
@@ -487,29 +529,21 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       // GradientSF of the static field (src/SELF_MappedData.f90:1080-1149)
       if constexpr (GRADN) {
         for (int ii = 0; ii < NP; ii++) {
-          gx = gx + Dm[ii + NP * i] * sGb[0][el][ii + NP * j] + Dm[ii + NP * j] * sGb[2][el][i + NP * ii];
-          gy = gy + Dm[ii + NP * i] * sGb[1][el][ii + NP * j] + Dm[ii + NP * j] * sGb[3][el][i + NP * ii];
+          gx = gx + Dm[ii + MRS * i] * sGb[0][el][slot(ii, j)] + Dm[ii + MRS * j] * sGb[2][el][slot(i, ii)];
+          gy = gy + Dm[ii + MRS * i] * sGb[1][el][slot(ii, j)] + Dm[ii + MRS * j] * sGb[3][el][slot(i, ii)];
         }
       } else {
         // affine element: the metric terms are constant, so they factor out of the sums
         double dxi = 0.0, deta = 0.0;
         for (int ii = 0; ii < NP; ii++) {
-          dxi = dxi + Dm[ii + NP * i] * sAux[el][ii + NP * j];
-          deta = deta + Dm[ii + NP * j] * sAux[el][i + NP * ii];
+          dxi = dxi + Dm[ii + MRS * i] * sAux[el][slot(ii, j)];
+          deta = deta + Dm[ii + MRS * j] * sAux[el][slot(i, ii)];
         }
         gx = dsdx[0] * dxi + dsdx[2] * deta;
         gy = dsdx[1] * dxi + dsdx[3] * deta;
       }
       gx = gx * invJ;
       gy = gy * invJ;
-    }
-    // the RK register is loaded here, not with the state: it is needed only for the update
-    if (MODE == 0 && a.rkA != 0.0) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = a.G[sBase + (size_t)v * NN + node];
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
     }
     double src[NV];
     Phys::source(s, gx, gy, statVal, a.p, src);
@@ -524,8 +558,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       double dxs = 0.0, des = 0.0;
 #pragma unroll
       for (int ii = 0; ii < NP; ii++) {
-        dxs = dxs + dgM[ii + NP * i] * sF1[v][el][ii + NP * j];
-        des = des + dgM[ii + NP * j] * sF2[v][el][i + NP * ii];
+        dxs = dxs + dgM[ii + MRS * i] * sF1[v][el][slot(ii, j)];
+        des = des + dgM[ii + MRS * j] * sF2[v][el][slot(i, ii)];
       }
       double d = dxs + des;
       d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) * iwi +
@@ -551,7 +585,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       double q[NT];
       phys_nodal<Phys>(s, a.p, q);
 #pragma unroll
-      for (int k = 0; k < NT; k++) sQ[k][el][node] = q[k];
+      for (int k = 0; k < NT; k++) sQ[k][el][nl] = q[k];
     }
   }
   if (MODE == 1) return;
@@ -561,23 +595,20 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   if constexpr (SPLIT) {
     if (faceActive) {
       const double *bm = (side == 1 || side == 2) ? bM1 : bM0;
-      const bool alongJ = (side == 0 || side == 2);
-      const int nbase = alongJ ? fi : NP * fi, nstride = alongJ ? NP : 1;
-      constexpr int H0 = NP / 2;
-      const int lo = fhalf ? H0 : 0;
+      const int lo = fhalf ? FH0 : 0;
       double qn[NT];
 #pragma unroll
       for (int k = 0; k < NT; k++) qn[k] = 0.0;
 #pragma unroll
-      for (int t = 0; t < NP - H0; t++) {
+      for (int t = 0; t < FHN; t++) {
         const int ii = lo + t;
-        if (fhalf == 0 && t >= H0) break;
-        const int nd = nbase + nstride * ii;
+        if (fhalf == 0 && t >= FH0) break;
+        const int nd = foff[t];
 #pragma unroll
         for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel][nd];
       }
 #pragma unroll
-      for (int k = 0; k < NT; k++) qn[k] += __shfl_xor(qn[k], 32);
+      for (int k = 0; k < NT; k++) qn[k] = pair_sum32(qn[k]);
       if (fhalf == 0) {
         double *tr = a.traceOut + (size_t)fi + NP * (size_t)NT * (side + 4 * (size_t)fe);
 #pragma unroll
@@ -594,13 +625,12 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     const int fe3 = a.elems ? a.elems[pf3] : pf3;
     const double *bm = (side3 == 1 || side3 == 2) ? bM1 : bM0;
     const bool alongJ = (side3 == 0 || side3 == 2);
-    const int nbase = alongJ ? fi3 : NP * fi3, nstride = alongJ ? NP : 1;
     double qn[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qn[k] = 0.0;
 #pragma unroll
     for (int ii = 0; ii < NP; ii++) {
-      const int nd = nbase + nstride * ii;
+      const int nd = alongJ ? slot(fi3, ii) : slot(ii, fi3);
 #pragma unroll
       for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel3][nd];
     }

@@ -537,10 +537,20 @@ __global__ void k_euler_riemann(double *__restrict__ fbn, const double *__restri
 }
 
 // Streaming copy dst = src (16-byte vector loads/stores): calibrates HBM counters and the
-// achievable bandwidth against kernels of known traffic.
+// achievable bandwidth against kernels of known traffic.  Each work-item issues four
+// independent loads before its stores, so that enough bytes are in flight per CU to cover
+// the HBM latency.
 __global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2) {
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
-    dst[t] = src[t];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (; t + 3 * stride < n2; t += 4 * stride) {
+    const double2 x0 = src[t], x1 = src[t + stride], x2 = src[t + 2 * stride], x3 = src[t + 3 * stride];
+    dst[t] = x0;
+    dst[t + stride] = x1;
+    dst[t + 2 * stride] = x2;
+    dst[t + 3 * stride] = x3;
+  }
+  for (; t < n2; t += stride) dst[t] = src[t];
 }
 
 inline unsigned blocks_for(size_t n) {
