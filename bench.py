@@ -57,7 +57,7 @@ def parse():
 def traffic_from_profiles(root, name):
     """Measured HBM bytes per stage launch, if a counter summary has been committed."""
     path = os.path.join(root, "profiles", name)
-    if not os.path.exists(path):
+    if not name or not os.path.isfile(path):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
@@ -189,6 +189,8 @@ def run_single(a, root):
     from self_amd import workloads
     w = setup_single(self_amd, workloads, a.workload, a.seed)
     model, dt, N = w["model"], w["dt"], w["N"]
+    # the benchmark meshes are affine: one metric record per element (raises if a mesh is not)
+    model.SetNodalMetrics(False)
     if a.traffic_probe:
         return traffic_probe(self_amd, model, dt)
     steps = a.steps or w["steps"]
@@ -212,14 +214,16 @@ def run_single(a, root):
         cpu = cpu_single(oracle, a.workload, w, a.cpu_steps or {"sw256": 100, "euler512": 5, "lsw10": 200}[a.workload])
     phys, kind = {"sw256": ("PhysSW", 2), "euler512": ("PhysEuler", 3), "lsw10": ("PhysLSW", 1)}[a.workload]
     kernel = tile_name(self_amd, N, phys, kind)
-    traffic = traffic_from_profiles(root, "sw2d_stage_256x256_N7_traffic.json") if a.workload == "sw256" else None
+    traffic = traffic_from_profiles(root, {"sw256": "sw2d_stage_256x256_N7_traffic.json",
+                                           "euler512": "euler2d_stage_512x512_N7_traffic.json"}.get(a.workload, ""))
     copy_gbs = copy_bandwidth(self_amd)
     out = {
         "metric": METRIC if a.workload == "sw256" else f"DOF-updates/sec + % HBM roofline, {a.workload}",
         "value": dof * steps / sec, "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall},
+        "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall,
+                   "metrics": "affine, one record per element"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "avg_launch_us": launch_s * 1e6, "bytes_per_launch_alg": bpd / 3.0 * dof,

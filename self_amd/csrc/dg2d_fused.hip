@@ -985,9 +985,12 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   SELFAMD_HIP(hipMemcpy(hJ.data(), J, sizeof(double) * nn, hipMemcpyDeviceToHost));
   SELFAMD_HIP(hipMemcpy(hn.data(), nHatb, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost));
   SELFAMD_HIP(hipMemcpy(hs.data(), nScaleb, sizeof(double) * nb, hipMemcpyDeviceToHost));
-  // Per-element affine records: metric terms constant over each element (to round-off, as
-  // the derivative of an interpolated bilinear map leaves ~1e-13 relative noise) use one
-  // record; otherwise the nodal metrics are used.
+  // Per-element affine records: metric terms constant over each element up to round-off use
+  // one record; otherwise the nodal metrics are used.  The round-off of a derivative of
+  // interpolated coordinates grows like (coordinate magnitude / element size) x eps: on the
+  // unit square it is ~3e-12 relative at 64 x 64 elements, ~1e-11 at 256 x 256 and ~2e-11 at
+  // 512 x 512, so the tolerance is 1e-10 (meshes up to ~2048 elements across).
+  constexpr double kAffineTol = 1e-10;
   std::vector<double> rec((size_t)kGeomStride * nEl, 0.0);
   bool affine = true;
   for (int e = 0; e < nEl && affine; e++) {
@@ -999,8 +1002,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
     r[6] = 1.0 / r[4];
     for (int n = 0; n < PP && affine; n++) {
       for (int k = 0; k < 4; k++)
-        if (std::fabs(d0[4 * n + k] - r[k]) > 1e-11 * scale) affine = false;
-      if (std::fabs(hJ[(size_t)PP * e + n] - r[4]) > 1e-11 * std::fabs(r[4])) affine = false;
+        if (std::fabs(d0[4 * n + k] - r[k]) > kAffineTol * scale) affine = false;
+      if (std::fabs(hJ[(size_t)PP * e + n] - r[4]) > kAffineTol * std::fabs(r[4])) affine = false;
     }
     for (int s = 0; s < 4 && affine; s++) {
       const size_t b0 = (size_t)P * (s + 4 * (size_t)e);
@@ -1008,8 +1011,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
       r[12 + s] = hn[2 * b0 + 1];
       r[16 + s] = hs[b0];
       for (int i = 1; i < P; i++)
-        if (std::fabs(hn[2 * (b0 + i)] - r[8 + s]) > 1e-11 || std::fabs(hn[2 * (b0 + i) + 1] - r[12 + s]) > 1e-11 ||
-            std::fabs(hs[b0 + i] - r[16 + s]) > 1e-11 * std::fabs(r[16 + s]))
+        if (std::fabs(hn[2 * (b0 + i)] - r[8 + s]) > kAffineTol || std::fabs(hn[2 * (b0 + i) + 1] - r[12 + s]) > kAffineTol ||
+            std::fabs(hs[b0 + i] - r[16 + s]) > kAffineTol * std::fabs(r[16 + s]))
           affine = false;
     }
   }
