@@ -287,7 +287,6 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // two lanes per face node (lanes l and l ^ 32 of a wavefront): each interpolates half of
   // the face line, the halves are summed across the pair
   constexpr bool SPLIT = ONE_FACE_PASS && 2 * EPB * NF <= THREADS && NF <= 32;
-  constexpr int FPW = SPLIT ? 32 : 64;  // face-node slots per wavefront
   static_assert(EPB >= 1, "block too small for one element");
   // LDS layouts.  Node arrays hold node (i, j) at slot(i, j).  With 8 x 8 nodes a row is 64
   // bytes, and a column access (fixed i, one row per lane group) would find rows 0, 2, 4, 6 in
@@ -1550,16 +1549,20 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
       if (e) return fail("copy failed");
     }
   }
-  // capture one RK3 step of every group, for both starting trace buffers
+  // capture one RK3 step of every group, for both starting trace buffers.  The groups are
+  // independent, so each group's three stages run back to back (a stage then starts on the
+  // data its predecessor left in the Infinity Cache), and the group order reverses between
+  // the two graphs, which alternate step by step, so that a group's last stage of one step is
+  // followed by its first stage of the next.
   for (int start = 0; start < 2; start++) {
     hipGraph_t gr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) return fail("capture failed");
     hipError_t le = hipSuccess;
-    int tin = start;
-    for (int st = 0; st < 3; st++) {
-      for (auto &g : b->groups)
-        if (le == hipSuccess) le = b->launch_group(g, st, tin);
-      tin ^= 1;
+    const int ng = (int)b->groups.size();
+    for (int k = 0; k < ng; k++) {
+      BatchGroup &g = b->groups[start == 0 ? k : ng - 1 - k];
+      for (int st = 0; st < 3; st++)
+        if (le == hipSuccess) le = b->launch_group(g, st, start ^ (st & 1));
     }
     hipError_t ce = hipStreamEndCapture(b->stream, &gr);
     if (le || ce) return fail("stage launch during capture failed");
