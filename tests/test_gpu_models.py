@@ -157,3 +157,15 @@ def test_full_size_sw_256():
     s = m.solution
     assert np.isfinite(s).all()
     assert abs(mass(s) - m0) <= 1e-13 * m0
+
+
+@pytest.mark.parametrize("nx", [512, 1024])
+def test_large_structured_mesh_is_affine(nx):
+    """The metric round-off of the device-generated SEMQuad geometry grows with mesh size
+    (~2e-11 relative at 512 x 512 on the unit square); the engine still takes such a mesh as
+    affine (one record per element), which SetNodalMetrics(False) requires."""
+    mesh = self_amd.Mesh2D.structured(nx, nx, bcid=self_amd.BC_NONORMALFLOW)
+    geom = self_amd.SEMQuad(self_amd.Lagrange(7, self_amd.GAUSS, 7, self_amd.GAUSS), mesh)
+    m = self_amd.CompressibleIdealGas2D(mesh, geom)
+    m.SetNodalMetrics(False)  # raises if the mesh was classified as not affine
+    m.SetNodalMetrics(True)
