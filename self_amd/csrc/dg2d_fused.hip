@@ -14,9 +14,10 @@
 // writes S, G and its own traces: the algorithmic minimum for a nodal DG stage with
 // Gauss points (no collocated face nodes).  Trace buffers alternate between launches.
 //
-// Work mapping: 256-wide blocks, EPB = 256/(N+1)^2 elements per block; at N = 7 one
-// wave64 owns one element, lane = i + 8 j, so every variable is one 512-byte coalesced
-// access and the per-element geometry record is wave-uniform.
+// Work mapping: blocks of 64, 128 or 256 work-items (the tile, autotuned per degree), EPB =
+// THREADS/(N+1)^2 elements per block; at N = 7 one wave64 owns one element, lane = i + 8 j,
+// so every variable is one 512-byte coalesced access and the per-element geometry record is
+// wave-uniform.
 //
 // Geometry: the fast path takes affine elements (metric terms constant per element, true
 // for the structured and block meshes); each element carries one 20-double record.
