@@ -1034,7 +1034,7 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   m->nVar = selfamd_model2d_nvar(kind);
   m->nT = kind == SELFAMD_MODEL_EULER ? 9 : 3;
   m->p.g = 1.0; m->p.fCori = 0.0; m->p.gamma = 1.4; m->p.Rgas = 1.0;
-  m->p.auxConst = kind == SELFAMD_MODEL_LSW ? 0.0 : 0.0;
+  m->p.auxConst = 0.0;  // static fields default to zero until a Set* call provides them
   m->p.stat2Const = 0.0;
   auto fail = [&](int code) { selfamd_model2d_destroy(m); return code; };
   if (stream) {

@@ -169,3 +169,31 @@ def test_large_structured_mesh_is_affine(nx):
     m = self_amd.CompressibleIdealGas2D(mesh, geom)
     m.SetNodalMetrics(False)  # raises if the mesh was classified as not affine
     m.SetNodalMetrics(True)
+
+
+def test_quiet_fluid_stays_at_rest():
+    """The reference's known-answer test, with its own setup (test/ShallowWater2D_QuietFluid.f90
+    and etc/cli/default.json): the Block2D HOPr mesh with radiation boundaries, N = 7 Gauss,
+    topography h = 1, a lake at rest with H = 1, RK3 with dt = 5e-4 to t = 0.01; the velocity
+    must stay at rest to 1000 eps (:13, :127-130)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hopr_block2d.json")) as f:
+        fx = json.load(f)
+    mesh = self_amd.Mesh2D.from_hopr(
+        fx["Ngeo"], np.array(fx["SideInfo"]["data"], np.int32).reshape(fx["SideInfo"]["shape"]),
+        np.array(fx["NodeCoords"]["data"]).reshape(fx["NodeCoords"]["shape"]),
+        np.array(fx["GlobalNodeIDs"]["data"], np.int32))
+    mesh.ResetBoundaryConditionType(self_amd.BC_RADIATION)
+    N = 7
+    geom = self_amd.SEMQuad(self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS), mesh)
+    m = self_amd.ShallowWater(mesh, geom)
+    m.SetTopography(np.ones((mesh.nElem, N + 1, N + 1)))
+    sol = np.zeros((mesh.nElem, 3, N + 1, N + 1))
+    sol[:, 2] = 1.0
+    m.SetSolution(sol)
+    m.ForwardStep(20, 5e-4)
+    s = m.solution
+    tol = 1000 * np.finfo(np.float64).eps
+    u, v = s[:, 0] / s[:, 2], s[:, 1] / s[:, 2]
+    assert np.abs(u).max() <= tol and np.abs(v).max() <= tol, (np.abs(u).max(), np.abs(v).max())
