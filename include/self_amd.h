@@ -283,6 +283,17 @@ int selfamd_model2d_set_solution(selfamd_model2d *model, const double *interior_
 int selfamd_model2d_get_solution(selfamd_model2d *model, double *interior_dev);
 /* CalculateTendency (src/SELF_Model2D.f90:816-859): dSdt(0:N,0:N,nVar,nEl) */
 int selfamd_model2d_tendency(selfamd_model2d *model, double *dSdt_dev);
+/* CalculateEntropy of the current state into *entropy (host): SW
+ * src/models/SELF_ShallowWater.f90:149-188, LSW src/models/SELF_LinearShallowWater.f90:171-216,
+ * Euler src/models/SELF_CompressibleIdealGas2D.f90:782-823.  Euler takes the pressure of the
+ * current state; the reference takes it from the primitive variables of its last tendency
+ * evaluation.  Deterministic reduction; synchronises the model's stream. */
+int selfamd_model2d_entropy(selfamd_model2d *model, double *entropy);
+/* sum over elements and nodes of w_i w_j J S(i,j,var,e) into *value (host): the mass of
+ * variable var (H for SW, eta for LSW, rho for Euler with var = 2).  No reference routine:
+ * the pattern of Integral_MappedScalar2D (src/SELF_MappedData.f90:1270-1311) with the
+ * variable as an argument. */
+int selfamd_model2d_integral(selfamd_model2d *model, int var, double *value);
 /* nSteps full low-storage RK3 steps of size dt (ForwardStep with tn = t + nSteps*dt). */
 int selfamd_model2d_forward_step(selfamd_model2d *model, int nSteps, double dt);
 /* Blocks until the model's stream has drained. */

@@ -278,6 +278,25 @@ class Model2D:
         s = _dev(interior, self.shape)
         call("selfamd_model2d_set_solution", self._h, _ptr(s))
 
+    def CalculateEntropy(self):
+        """CalculateEntropy (SW src/models/SELF_ShallowWater.f90:149-188, LSW
+        src/models/SELF_LinearShallowWater.f90:171-216, Euler
+        src/models/SELF_CompressibleIdealGas2D.f90:782-823) of the current state, computed on
+        the device; also stored as `self.entropy`.  Euler uses the pressure of the current
+        state (the reference: that of its last tendency evaluation)."""
+        v = ctypes.c_double()
+        call("selfamd_model2d_entropy", self._h, ctypes.byref(v))
+        self.entropy = v.value
+        return v.value
+
+    def Integral(self, var):
+        """sum over elements and nodes of w_i w_j J S(i, j, var, e), on the device (the mass of
+        variable `var`; no reference routine, the pattern of Integral_MappedScalar2D,
+        src/SELF_MappedData.f90:1270-1311)."""
+        v = ctypes.c_double()
+        call("selfamd_model2d_integral", self._h, int(var), ctypes.byref(v))
+        return v.value
+
     @property
     def solution(self):
         """Host copy of the solution interior[e, v, j, i]."""

@@ -77,6 +77,8 @@ SIGNATURES = {
     "selfamd_model2d_set_solution": [c_void_p, c_void_p],
     "selfamd_model2d_get_solution": [c_void_p, c_void_p],
     "selfamd_model2d_tendency": [c_void_p, c_void_p],
+    "selfamd_model2d_entropy": [c_void_p, c_void_p],
+    "selfamd_model2d_integral": [c_void_p, c_int, c_void_p],
     "selfamd_model2d_forward_step": [c_void_p, c_int, c_double],
     "selfamd_model2d_synchronize": [c_void_p],
     "selfamd_model2d_time_steps": [c_void_p, c_int, c_double, c_void_p],

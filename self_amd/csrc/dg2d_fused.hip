@@ -644,6 +644,68 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   }
 }
 
+// ------------------------------------------------------------------------------
+// Integrals of the state: CalculateEntropy and mass-type integrals
+// ------------------------------------------------------------------------------
+// One work-item per node forms w_i w_j J f(S) (the per-node J as SEMQuad stores it); blocks
+// sum in a fixed tree order into partial sums, and one block adds those in a fixed order, so
+// the result does not depend on scheduling.
+constexpr int kIntBlock = 256;
+enum IntegralKind { kIntSWEntropy = 0, kIntLSWEntropy = 1, kIntEulerEntropy = 2, kIntVariable = 3 };
+
+__device__ inline void block_sum_store(double v, double *out) {
+  __shared__ double red[kIntBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = kIntBlock / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+template <int WHAT>
+__global__ void __launch_bounds__(kIntBlock) k_integral_partial(const double *S, const double *aux, const double *J,
+                                                                const double *qW, Params p, int N, int nVar, int var,
+                                                                size_t nNodes, double *partial) {
+  const size_t n = blockIdx.x * (size_t)kIntBlock + threadIdx.x;
+  double v = 0.0;
+  if (n < nNodes) {
+    const int P = N + 1, PP = P * P;
+    const size_t e = n / PP;
+    const int node = (int)(n - e * PP), i = node % P, j = node / P;
+    const double jac = J[n], wi = qW[i], wj = qW[j];
+    const double *s = S + e * (size_t)nVar * PP + node;  // variable k at s[k * PP]
+    if (WHAT == kIntSWEntropy) {
+      // CalculateEntropy_ShallowWater (src/models/SELF_ShallowWater.f90:149-188)
+      const double Hu = s[0], Hv = s[PP], H = s[2 * PP], b = aux ? aux[n] : p.auxConst;
+      v = (0.5 * (Hu * Hu / H + Hv * Hv / H) + 0.5 * p.g * H * H - p.g * H * b) * jac * wi * wj;
+    } else if (WHAT == kIntLSWEntropy) {
+      // CalculateEntropy_LinearShallowWater (src/models/SELF_LinearShallowWater.f90:171-216)
+      const double u = s[0], vv = s[PP], eta = s[2 * PP], H = aux ? aux[n] : p.auxConst;
+      v = 0.5 * (H * (u * u + vv * vv) + p.g * eta * eta) * jac * wi * wj;
+    } else if (WHAT == kIntEulerEntropy) {
+      // CalculateEntropy_CompressibleIdealGas2D (src/models/SELF_CompressibleIdealGas2D.f90:782-823)
+      // with the pressure of the current state (ConservativeToPrimitive, :1129-1170)
+      const double rhoU = s[0], rhoV = s[PP], rho = s[2 * PP], rhoE = s[3 * PP];
+      const double u = rhoU / rho, vv = rhoV / rho;
+      const double pr = (p.gamma - 1.0) * (rhoE - 0.5 * (rhoU * u + rhoV * vv));
+      v = rho * (log(pr) - p.gamma * log(rho)) / (p.gamma - 1.0) * wi * wj * jac;
+    } else {
+      v = s[(size_t)var * PP] * wi * wj * jac;
+    }
+  }
+  block_sum_store(v, partial + blockIdx.x);
+}
+
+// one block: work-item t adds partial[t], partial[t + kIntBlock], ... in order, then the
+// block sums the work-items' totals in a fixed tree order
+__global__ void __launch_bounds__(kIntBlock) k_sum_partials(const double *partial, int n, double *out) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += kIntBlock) v += partial[k];
+  block_sum_store(v, out);
+}
+
 // Traces of the nodal quantities of the current state (SetSolution's BoundaryInterp).
 template <class Phys, int N>
 __global__ void __launch_bounds__(kThreads) k_traces(const double *S, double *traceOut, const double *mats, Params p,
@@ -839,6 +901,8 @@ struct selfamd_model2d {
   double graphDt = -1.0;
   int graphEpoch = -1;  // tile table version the graphs were captured with
   ClosureBuffers *closure = nullptr;
+  double *intScratch = nullptr;  // partial sums of the state integrals (allocated on first use)
+  size_t intScratchLen = 0;
 
   // Banded stepping: the elements split into bands of whole BFS levels of the element graph
   // (neighbours differ by at most one band); one RK3 step runs stage s of band b in round
@@ -1093,6 +1157,7 @@ int selfamd_model2d_destroy(selfamd_model2d *m) {
   double *bufs[] = {m->S,    m->G,    m->trace[0], m->trace[1], m->un[0], m->un[1], m->geom,  m->mats,
                     m->aux,  m->stat2, m->presTrace, m->ndsdx, m->nJ,    m->nnHat, m->nnScale};
   if (m->bandElems) (void)hipFree(m->bandElems);
+  if (m->intScratch) (void)hipFree(m->intScratch);
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (m->nbr) (void)hipFree(m->nbr);
@@ -1162,6 +1227,54 @@ int selfamd_model2d_get_solution(selfamd_model2d *m, double *interior) {
   SELFAMD_HIP(hipMemcpyAsync(interior, m->S, sizeof(double) * n, hipMemcpyDeviceToDevice, m->stream));
   SELFAMD_HIP(hipStreamSynchronize(m->stream));
   return kOk;
+}
+
+// integral of the current state over the mesh (the integrand kind, and the variable for
+// kIntVariable); synchronises the model's stream
+static int model_integral(selfamd_model2d *m, int what, int var, double *value, const char *who) {
+  const size_t PP = (size_t)(m->N + 1) * (m->N + 1), nNodes = PP * m->nEl;
+  const int nBlocks = (int)((nNodes + kIntBlock - 1) / kIntBlock);
+  if (m->intScratchLen < (size_t)nBlocks + 1) {
+    if (m->intScratch) SELFAMD_HIP(hipFree(m->intScratch));
+    m->intScratch = nullptr;
+    m->intScratchLen = 0;
+    if (hipMalloc(&m->intScratch, sizeof(double) * (nBlocks + 1)) != hipSuccess) {
+      set_error("%s: device allocation failed", who);
+      return kHipError;
+    }
+    m->intScratchLen = (size_t)nBlocks + 1;
+  }
+  const double *qW = m->mats + 2 * PP + 2 * (m->N + 1);
+  switch (what) {
+#define INT_CASE(W)                                                                                        \
+  case W:                                                                                                 \
+    hipLaunchKernelGGL((k_integral_partial<W>), dim3(nBlocks), dim3(kIntBlock), 0, m->stream, m->S, m->aux, \
+                       m->nJ, qW, m->p, m->N, m->nVar, var, nNodes, m->intScratch);                         \
+    break;
+    INT_CASE(kIntSWEntropy) INT_CASE(kIntLSWEntropy) INT_CASE(kIntEulerEntropy) INT_CASE(kIntVariable)
+#undef INT_CASE
+    default: set_error("%s: unknown integral", who); return kBadArgument;
+  }
+  SELFAMD_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kIntBlock), 0, m->stream, m->intScratch, nBlocks,
+                     m->intScratch + nBlocks);
+  SELFAMD_CHECK_LAUNCH();
+  SELFAMD_HIP(hipMemcpyAsync(value, m->intScratch + nBlocks, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+int selfamd_model2d_entropy(selfamd_model2d *m, double *entropy) {
+  SELFAMD_REQUIRE(m && entropy, "model2d_entropy: null argument");
+  const int what = m->kind == SELFAMD_MODEL_SW ? kIntSWEntropy
+                   : m->kind == SELFAMD_MODEL_LSW ? kIntLSWEntropy : kIntEulerEntropy;
+  return model_integral(m, what, 0, entropy, "model2d_entropy");
+}
+
+int selfamd_model2d_integral(selfamd_model2d *m, int var, double *value) {
+  SELFAMD_REQUIRE(m && value, "model2d_integral: null argument");
+  SELFAMD_REQUIRE(var >= 0 && var < m->nVar, "model2d_integral: variable %d out of range (nVar = %d)", var, m->nVar);
+  return model_integral(m, kIntVariable, var, value, "model2d_integral");
 }
 
 int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {

@@ -24,6 +24,7 @@ MODULE SELF_Model2D
     INTEGER :: kind = 0
     REAL(prec) :: t = 0.0_prec
     REAL(prec) :: dt = 0.0_prec
+    REAL(prec) :: entropy = 0.0_prec
     TYPE(MappedScalar2D) :: solution
     TYPE(MappedScalar2D) :: dSdt
     TYPE(Mesh2D), POINTER :: mesh => NULL()
@@ -37,6 +38,9 @@ MODULE SELF_Model2D
     PROCEDURE, PUBLIC :: CalculateTendency => CalculateTendency_Model2D
     PROCEDURE, PUBLIC :: ForwardStep => ForwardStep_Model2D
     PROCEDURE, PUBLIC :: UpdateHost => UpdateHost_Model2D
+    PROCEDURE, PUBLIC :: CalculateEntropy => CalculateEntropy_Model2D
+    PROCEDURE, PUBLIC :: ReportEntropy => ReportEntropy_Model2D
+    PROCEDURE, PUBLIC :: Integral => Integral_Model2D
     PROCEDURE, PUBLIC :: SetNodalMetrics => SetNodalMetrics_Model2D
     PROCEDURE, PRIVATE :: SetField => SetField_Model2D
   END TYPE Model2D
@@ -171,6 +175,34 @@ CONTAINS
     CALL hip_check(selfamd_memcpy(C_LOC(this % dSdt % interior), this % dSdt % interior_dev, &
                    INT(SIZE(this % dSdt % interior), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), 'selfamd_memcpy')
   END SUBROUTINE UpdateHost_Model2D
+
+  ! CalculateEntropy (src/models/SELF_ShallowWater.f90:149-188, SELF_LinearShallowWater.f90:171-216,
+  ! SELF_CompressibleIdealGas2D.f90:782-823), reduced on the device
+  SUBROUTINE CalculateEntropy_Model2D(this)
+    CLASS(Model2D), INTENT(inout) :: this
+    REAL(C_DOUBLE) :: e
+    CALL selfamd_check(selfamd_model2d_entropy(this % engine, e), 'Model2D%CalculateEntropy')
+    this % entropy = e
+  END SUBROUTINE CalculateEntropy_Model2D
+
+  ! ReportEntropy_Model (src/SELF_Model.f90:440-469): time and entropy on standard output, in
+  ! the reference's ES16.7E3 format (ASCII labels)
+  SUBROUTINE ReportEntropy_Model2D(this)
+    CLASS(Model2D), INTENT(in) :: this
+    CHARACTER(LEN=20) :: modelTime, entropy
+    WRITE (modelTime, "(ES16.7E3)") this % t
+    WRITE (entropy, "(ES16.7E3)") this % entropy
+    WRITE (*, '(A)') 't_i =' // TRIM(modelTime) // '  |  e_i =' // TRIM(entropy)
+  END SUBROUTINE ReportEntropy_Model2D
+
+  ! sum of w_i w_j J S(i,j,var) over the mesh (var counted from 1, as in the reference)
+  REAL(prec) FUNCTION Integral_Model2D(this, var) RESULT(v)
+    CLASS(Model2D), INTENT(in) :: this
+    INTEGER, INTENT(in) :: var
+    REAL(C_DOUBLE) :: r
+    CALL selfamd_check(selfamd_model2d_integral(this % engine, INT(var - 1, C_INT), r), 'Model2D%Integral')
+    v = r
+  END FUNCTION Integral_Model2D
 
   SUBROUTINE SetField_Model2D(this, which, values)
     CLASS(Model2D), INTENT(inout) :: this

@@ -73,11 +73,18 @@ CONTAINS
 
   SUBROUTINE run_model(m)
     CLASS(Model2D), INTENT(inout) :: m
+    INTEGER :: v
     CALL m % SetTimeIntegrator('rk3')
     CALL m % SetSolution(sol)
     CALL m % CalculateTendency()
     CALL m % UpdateHost()
     CALL write_array(TRIM(prefix)//'_tend.bin', m % dSdt % interior, SIZE(m % dSdt % interior))
+    ! entropy and mass (H, eta or rho: variable 3) of the initial state
+    CALL m % CalculateEntropy()
+    CALL m % ReportEntropy()
+    OPEN (NEWUNIT=v, FILE=TRIM(prefix)//'_diag.txt', STATUS='replace', ACTION='write')
+    WRITE (v, '(2ES26.17)') m % entropy, m % Integral(3)
+    CLOSE (v)
     m % t = 0.0_prec
     CALL m % ForwardStep(tn=REAL(nSteps, prec)*dt, dt=dt)
     CALL m % UpdateHost()

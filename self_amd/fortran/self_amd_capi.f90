@@ -172,6 +172,17 @@ MODULE SELF_AMD_CAPI
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: model, dSdt
     END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_entropy(model, entropy) BIND(C, name="selfamd_model2d_entropy")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: model
+      REAL(C_DOUBLE) :: entropy
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_integral(model, var, val) BIND(C, name="selfamd_model2d_integral")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: model
+      INTEGER(C_INT), VALUE :: var
+      REAL(C_DOUBLE) :: val
+    END FUNCTION
     INTEGER(C_INT) FUNCTION selfamd_model2d_forward_step(model, nSteps, dt) &
         BIND(C, name="selfamd_model2d_forward_step")
       IMPORT :: C_INT, C_PTR, C_DOUBLE

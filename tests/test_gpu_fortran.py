@@ -36,5 +36,10 @@ def test_fortran_driver_parity(kind, model, N, n, periodic, bcid, tmp_path):
     tend = np.fromfile(prefix + "_tend.bin").reshape(shape)
     out = np.fromfile(prefix + "_out.bin").reshape(shape)
     assert rel_err(tend, c.om.tendency()).max() <= TOL
+    # CalculateEntropy / Integral of the initial state through the Fortran layer (the oracle's
+    # Euler entropy takes the pressure of its last tendency evaluation: the same state here)
+    entropy, mass = np.loadtxt(prefix + "_diag.txt")
+    assert abs(entropy - c.om.entropy()) <= TOL * abs(c.om.entropy())
+    assert abs(mass - c.om.mass(2)) <= TOL * abs(c.om.mass(2))
     c.om.rk3_steps(steps, dt)
     assert rel_err(out, c.om.solution).max() <= TOL

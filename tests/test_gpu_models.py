@@ -197,3 +197,24 @@ def test_quiet_fluid_stays_at_rest():
     tol = 1000 * np.finfo(np.float64).eps
     u, v = s[:, 0] / s[:, 2], s[:, 1] / s[:, 2]
     assert np.abs(u).max() <= tol and np.abs(v).max() <= tol, (np.abs(u).max(), np.abs(v).max())
+
+
+@pytest.mark.parametrize("kind,N,n,periodic", [("sw", 7, 6, True), ("sw", 3, 5, False), ("lsw", 5, 5, False),
+                                               ("euler", 4, 4, False), ("euler", 7, 3, True)])
+def test_entropy_and_integrals_match_oracle(kind, N, n, periodic):
+    """CalculateEntropy and the variable integrals, reduced on the device, against the oracle
+    (or_entropy, or_mass) on the same state, before and after RK3 steps."""
+    c = Case(kind, N, n, n, periodic=periodic, seed=3, metrics="nodal")
+    m, _ = c.gpu_model()
+    w = c.weights
+    for step in range(2):
+        if step:
+            m.ForwardStep(3, c.dt())
+            c.om.rk3_steps(3, c.dt())
+        c.om.tendency()  # the oracle's Euler entropy takes the pressure of its last tendency evaluation
+        e, ref = m.CalculateEntropy(), c.om.entropy()
+        assert abs(e - ref) <= TOL * abs(ref), (e, ref)
+        s = c.om.solution
+        for v in range(c.om.nVar):
+            scale = np.sum(np.abs(s[:, v]) * c.geom.J * w[None, :, None] * w[None, None, :])
+            assert abs(m.Integral(v) - c.om.mass(v)) <= TOL * scale

@@ -354,3 +354,53 @@ def test_sw_velocity_lag_first_order_in_time():
     e = [np.abs(s - ref).max() for s in sols[:3]]
     rates = np.log2(np.array(e[:-1]) / np.array(e[1:]))
     assert np.all((rates > 0.8) & (rates < 1.5)), rates
+
+
+def _constant_case(kind, N, n, sol_vals, fields):
+    c = Case(kind, N, n, n, periodic=True, generic=False)
+    nel, P = c.mesh.nElem, N + 1
+    c.sol = np.zeros((nel, len(sol_vals), P, P))
+    for v, val in enumerate(sol_vals):
+        c.sol[:, v] = val
+    for k, val in fields.items():
+        c.fields[k] = np.full((nel, P, P), float(val))
+    c.apply_oracle(c.om)
+    return c
+
+
+def test_entropy_and_mass_closed_forms():
+    """or_entropy (CalculateEntropy of each model) and or_mass on constant states of the unit
+    square, where the Gauss quadrature is exact and the integrals are closed forms."""
+    g, gam = 1.0, 1.4
+    # SW (src/models/SELF_ShallowWater.f90:149-188): 1/2 (Hu^2 + Hv^2)/H + 1/2 g H^2 - g H b
+    H0, u0, v0, b0 = 1.3, 0.2, -0.1, 0.05
+    c = _constant_case("sw", 5, 3, (H0 * u0, H0 * v0, H0), {"topo": b0})
+    want = 0.5 * (H0 * u0 ** 2 + H0 * v0 ** 2) + 0.5 * g * H0 ** 2 - g * H0 * b0
+    assert abs(c.om.entropy() - want) <= 1e-14 * abs(want)
+    assert abs(c.om.mass(2) - H0) <= 1e-14 * H0
+    assert abs(c.om.mass(0) - H0 * u0) <= 1e-14 * H0
+    # LSW (src/models/SELF_LinearShallowWater.f90:171-216): 1/2 (H (u^2 + v^2) + g eta^2)
+    Hb, eta0 = 0.7, 0.03
+    c = _constant_case("lsw", 4, 3, (u0, v0, eta0), {"H": Hb, "f": 0.0})
+    want = 0.5 * (Hb * (u0 ** 2 + v0 ** 2) + g * eta0 ** 2)
+    assert abs(c.om.entropy() - want) <= 1e-14 * abs(want)
+    assert abs(c.om.mass(2) - eta0) <= 1e-14 * eta0
+    # Euler (src/models/SELF_CompressibleIdealGas2D.f90:782-823): rho (ln p - gamma ln rho) / (gamma - 1)
+    rho0, p0 = 1.2, 0.9
+    c = _constant_case("euler", 3, 3, (rho0 * u0, rho0 * v0, rho0, p0 / (gam - 1) + 0.5 * rho0 * (u0 ** 2 + v0 ** 2)),
+                       {"phi": 0.0, "drag": 0.0})
+    c.om.tendency()  # the pressure is the primitive variable of the last tendency evaluation
+    want = rho0 * (np.log(p0) - gam * np.log(rho0)) / (gam - 1)
+    assert abs(c.om.entropy() - want) <= 1e-13 * abs(want)
+    assert abs(c.om.mass(2) - rho0) <= 1e-14 * rho0
+
+
+def test_mass_of_a_polynomial_is_exact():
+    """A state linear in x integrates exactly: H = 1 + 0.1 x over the unit square is 1.05."""
+    c = Case("sw", 3, 4, 4, periodic=True, generic=False)
+    x = c.geom.x[..., 0]
+    c.sol = np.zeros_like(c.sol)
+    c.sol[:, 2] = 1.0 + 0.1 * x
+    c.fields["topo"] = np.zeros_like(x)
+    c.apply_oracle(c.om)
+    assert abs(c.om.mass(2) - 1.05) <= 1e-14
