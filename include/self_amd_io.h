@@ -1,0 +1,42 @@
+/* self_amd_io.h -- C ABI of libself_amd_io.so: reading HOPr mesh files (HDF5).
+ *
+ * Replaces the file access of Read_HOPr_Mesh2D (src/SELF_Mesh.f90:852-1016), which opens the
+ * mesh file (Open_HDF5_serial, src/SELF_HDF5.f90:149-170) and reads the attributes nElems,
+ * Ngeo, nBCs, nUniqueSides and the datasets ElemInfo, SideInfo, NodeCoords, GlobalNodeIDs
+ * and BCType (ReadAttribute_HDF5_int32 :226-246, the serial ReadArray_HDF5 variants).  The
+ * 3-D -> 2-D conversion that follows in the reference is selfamd_mesh2d_from_hopr (self_amd.h).
+ *
+ * A separate library so that libself_amd.so carries no HDF5 dependency.  Arrays are returned
+ * in the file's row-major (C) order, e.g. SideInfo as (nSides, 5), NodeCoords as (nNodes, 3).
+ * Every entry point returns 0 on success, 1 for a bad argument and 5 for a file or HDF5
+ * error; selfamd_io_last_error() describes the last failure.
+ */
+#ifndef SELF_AMD_IO_H
+#define SELF_AMD_IO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *selfamd_io_last_error(void);
+
+/* Integer attribute of the file's root group (ReadAttribute_HDF5_int32,
+ * src/SELF_HDF5.f90:226-246). */
+int selfamd_hopr_attribute_int(const char *path, const char *name, int32_t *value);
+
+/* Rank (1 or 2) and dimensions (row-major, dims[0] slowest) of a dataset. */
+int selfamd_hopr_dataset_dims(const char *path, const char *name, int32_t *rank, int64_t *dims);
+
+/* A whole dataset converted to int32 / double (the serial ReadArray_HDF5 variants,
+ * e.g. ReadArray_HDF5_real_r2_serial, src/SELF_HDF5.f90:1928-1946); count is the number of
+ * elements the caller's buffer holds and must equal the dataset's size. */
+int selfamd_hopr_read_int(const char *path, const char *name, int32_t *out, int64_t count);
+int selfamd_hopr_read_double(const char *path, const char *name, double *out, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SELF_AMD_IO_H */

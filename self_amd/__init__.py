@@ -12,6 +12,7 @@ reversed, a[e, v, j, i]; boundary(0:N,nVar,4,nEl) is b[e, s, v, i]; sideInfo(5,4
 si[e, s, k].
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -175,6 +176,29 @@ class Mesh2D:
         m = cls(si, nc, gn, 1)
         m.nx, m.ny = nx, ny
         return m
+
+    @classmethod
+    def read_hopr(cls, path):
+        """Read_HOPr_Mesh2D (src/SELF_Mesh.f90:852-1016) from a HOPr mesh file: the attribute
+        Ngeo and the datasets SideInfo, NodeCoords and GlobalNodeIDs are read with the HDF5 C
+        library (libself_amd_io.so), then converted by from_hopr."""
+        from ._lib import call_io
+        p = os.fsencode(os.path.abspath(path))
+        ngeo = ctypes.c_int32()
+        call_io("selfamd_hopr_attribute_int", p, b"Ngeo", ctypes.byref(ngeo))
+
+        def read(name, dtype, fn):
+            rank, dims = ctypes.c_int32(), (ctypes.c_int64 * 2)()
+            call_io("selfamd_hopr_dataset_dims", p, name, ctypes.byref(rank), dims)
+            shape = tuple(dims[k] for k in range(rank.value))
+            a = np.empty(shape, dtype)
+            call_io(fn, p, name, a.ctypes.data, a.size)
+            return a
+
+        sideInfo = read(b"SideInfo", np.int32, "selfamd_hopr_read_int")
+        nodeCoords = read(b"NodeCoords", np.float64, "selfamd_hopr_read_double")
+        globalNodeIDs = read(b"GlobalNodeIDs", np.int32, "selfamd_hopr_read_int").reshape(-1)
+        return cls.from_hopr(ngeo.value, sideInfo, nodeCoords, globalNodeIDs)
 
     @classmethod
     def from_hopr(cls, nGeo, sideInfo3D, nodeCoords3D, globalNodeIDs3D):

@@ -9,6 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libself_amd.so")
+IO_LIB_PATH = os.path.join(_HERE, "libself_amd_io.so")
 
 c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 c_i32p = ctypes.POINTER(ctypes.c_int32)
@@ -142,4 +143,40 @@ def call(name, *args):
     if rc != 0:
         msg = lib.selfamd_last_error().decode(errors="replace")
         raise SelfAmdError(f"{name} failed (status {rc}): {msg}")
+    return rc
+
+
+# libself_amd_io.so (include/self_amd_io.h): HOPr mesh files through the HDF5 C library
+IO_SIGNATURES = {
+    "selfamd_io_last_error": [],
+    "selfamd_hopr_attribute_int": [ctypes.c_char_p, ctypes.c_char_p, c_void_p],
+    "selfamd_hopr_dataset_dims": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_void_p],
+    "selfamd_hopr_read_int": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
+    "selfamd_hopr_read_double": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
+}
+_io = None
+
+
+def load_io():
+    """Load libself_amd_io.so once; ImportError when it was not built (no HDF5 at build time)."""
+    global _io
+    if _io is not None:
+        return _io
+    if not os.path.exists(IO_LIB_PATH):
+        raise ImportError(f"self_amd HOPr reader not found at {IO_LIB_PATH}; it is built by "
+                          "`make -C self_amd/csrc` where the HDF5 C library is installed")
+    lib = ctypes.CDLL(IO_LIB_PATH)
+    for name, argtypes in IO_SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_char_p if name == "selfamd_io_last_error" else c_int
+    _io = lib
+    return lib
+
+
+def call_io(name, *args):
+    lib = load_io()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise SelfAmdError(f"{name} failed (status {rc}): {lib.selfamd_io_last_error().decode(errors='replace')}")
     return rc

@@ -23,8 +23,9 @@ MESHES = {
 
 
 def _dump(path, kind, name):
-    out = subprocess.run([H5DUMP, "-y", "-w", "0", kind, name, path], check=True, capture_output=True,
-                         text=True).stdout
+    # %.17g: every double printed with enough digits to read back bit-exactly
+    out = subprocess.run([H5DUMP, "-y", "-w", "0", "-m", "%.17g", kind, name, path], check=True,
+                         capture_output=True, text=True).stdout
     dims = re.search(r"DATASPACE\s+SIMPLE \{ \( ([^)]*) \)", out).group(1)
     shape = [int(d) for d in dims.split(",")]
     body = out[out.index("DATA {") + 6: out.rindex("}", 0, out.rindex("}"))]

@@ -26,6 +26,21 @@ def test_every_declared_symbol_is_exported_and_bound():
     assert set(_lib.SIGNATURES) == set(names)
 
 
+IO_HEADER = os.path.join(os.path.dirname(HEADER), "self_amd_io.h")
+
+
+def test_io_library_exports_its_header():
+    if not os.path.exists(_lib.IO_LIB_PATH):
+        import pytest
+        pytest.skip("libself_amd_io.so not built (no HDF5 at build time)")
+    src = re.sub(r"/\*.*?\*/", "", open(IO_HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(selfamd_\w+)\s*\(", src)))
+    lib = ctypes.CDLL(_lib.IO_LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_lib.IO_SIGNATURES) == set(names)
+
+
 def test_version_and_error_channel():
     lib = self_amd.load()
     assert lib.selfamd_version() == 1

@@ -218,3 +218,12 @@ def test_entropy_and_integrals_match_oracle(kind, N, n, periodic):
         for v in range(c.om.nVar):
             scale = np.sum(np.abs(s[:, v]) * c.geom.J * w[None, :, None] * w[None, None, :])
             assert abs(m.Integral(v) - c.om.mass(v)) <= TOL * scale
+
+
+def test_integral_rejects_bad_variable():
+    c = Case("sw", 3, 2, 2, periodic=True, seed=1, metrics="affine")
+    m, _ = c.gpu_model()
+    with pytest.raises(Exception, match="out of range"):
+        m.Integral(3)
+    with pytest.raises(Exception, match="out of range"):
+        m.Integral(-1)
