@@ -8,6 +8,9 @@
  *
  * A separate library so that libself_amd.so carries no HDF5 dependency.  Arrays are returned
  * in the file's row-major (C) order, e.g. SideInfo as (nSides, 5), NodeCoords as (nNodes, 3).
+ * Also the model pickup files of WriteModel / ReadModel (Write_Model2D / Read_Model2D,
+ * src/SELF_Model2D.f90:861-1099).
+ *
  * Every entry point returns 0 on success, 1 for a bad argument and 5 for a file or HDF5
  * error; selfamd_io_last_error() describes the last failure.
  */
@@ -26,7 +29,7 @@ const char *selfamd_io_last_error(void);
  * src/SELF_HDF5.f90:226-246). */
 int selfamd_hopr_attribute_int(const char *path, const char *name, int32_t *value);
 
-/* Rank (1 or 2) and dimensions (row-major, dims[0] slowest) of a dataset. */
+/* Rank (1 to 5) and dimensions (row-major, dims[0] slowest) of a dataset; dims holds 5. */
 int selfamd_hopr_dataset_dims(const char *path, const char *name, int32_t *rank, int64_t *dims);
 
 /* A whole dataset converted to int32 / double (the serial ReadArray_HDF5 variants,
@@ -34,6 +37,22 @@ int selfamd_hopr_dataset_dims(const char *path, const char *name, int32_t *rank,
  * elements the caller's buffer holds and must equal the dataset's size. */
 int selfamd_hopr_read_int(const char *path, const char *name, int32_t *out, int64_t count);
 int selfamd_hopr_read_double(const char *path, const char *name, double *out, int64_t count);
+
+/* WriteModel (Write_Model2D, src/SELF_Model2D.f90:861-1059, serial branch): host arrays in the
+ * reference's Fortran layouts, written with their dimensions reversed as the reference's HDF5
+ * writes show them: /quadrature/{xi, weights, dgmatrix, dmatrix, bmatrix},
+ * /state/interior/solution (0:N,0:N,nVar,nEl), /state/boundary/solution (0:N,nVar,4,nEl),
+ * /mesh/interior/x (2,0:N,0:N,1,nEl), /mesh/boundary/x (2,0:N,1,4,nEl), plus the root
+ * attribute N, which Read_Model2D reads (:1078) and the reference never writes.  The
+ * fluxDivergence, flux and solutionGradient datasets are not written: the fused engine does
+ * not form them, and ReadModel does not read them.  An existing file is replaced. */
+int selfamd_pickup_write(const char *path, int N, int nVar, int nEl, const double *xi, const double *weights,
+                         const double *dgMatrix, const double *dMatrix, const double *bMatrix, const double *solution,
+                         const double *solutionBoundary, const double *x, const double *xBoundary);
+
+/* ReadModel (Read_Model2D, src/SELF_Model2D.f90:1061-1099): checks the attribute N against N
+ * and the solution's dimensions, then reads /state/interior/solution (0:N,0:N,nVar,nEl). */
+int selfamd_pickup_read(const char *path, int N, int nVar, int nEl, double *solution);
 
 #ifdef __cplusplus
 }

@@ -188,7 +188,7 @@ class Mesh2D:
         call_io("selfamd_hopr_attribute_int", p, b"Ngeo", ctypes.byref(ngeo))
 
         def read(name, dtype, fn):
-            rank, dims = ctypes.c_int32(), (ctypes.c_int64 * 2)()
+            rank, dims = ctypes.c_int32(), (ctypes.c_int64 * 5)()
             call_io("selfamd_hopr_dataset_dims", p, name, ctypes.byref(rank), dims)
             shape = tuple(dims[k] for k in range(rank.value))
             a = np.empty(shape, dtype)
@@ -320,6 +320,36 @@ class Model2D:
         v = ctypes.c_double()
         call("selfamd_model2d_integral", self._h, int(var), ctypes.byref(v))
         return v.value
+
+    def WriteModel(self, path):
+        """WriteModel (Write_Model2D, src/SELF_Model2D.f90:861-1059) to an HDF5 pickup file:
+        quadrature, interior and boundary solution, interior and boundary mesh positions, and
+        the attribute N that ReadModel needs (libself_amd_io.so).  The geometry must carry x
+        and xb (a SEMQuad)."""
+        from ._lib import call_io
+        interp, g = self.geometry.interp, self.geometry
+        sol = self.solution
+        dev = DeviceArray.from_host(sol)
+        bnd = DeviceArray((self.nElem, 4, self.nVar, self.N + 1))
+        bm = DeviceArray.from_host(interp.bMatrix)
+        call("selfamd_scalar_boundary_interp_2d", bm.ptr, dev.ptr, bnd.ptr, self.N, self.nVar, self.nElem, None)
+        synchronize()
+        x, xb = g.x.to_host(), g.xb.to_host()
+        host = [np.ascontiguousarray(a, np.float64) for a in (interp.controlPoints, interp.qWeights, interp.dgMatrix,
+                                                               interp.dMatrix, interp.bMatrix, sol, bnd.to_host(), x, xb)]
+        call_io("selfamd_pickup_write", os.fsencode(os.path.abspath(path)), self.N, self.nVar, self.nElem,
+                *[a.ctypes.data for a in host])
+
+    def ReadModel(self, path):
+        """ReadModel (Read_Model2D, src/SELF_Model2D.f90:1061-1099): the solution of a pickup
+        file (its N must match), set with SetSolution.  The SW velocity lag restarts from the
+        read state, as in the reference, whose pickup holds no lag state either."""
+        from ._lib import call_io
+        sol = np.empty(self.shape)
+        call_io("selfamd_pickup_read", os.fsencode(os.path.abspath(path)), self.N, self.nVar, self.nElem,
+                sol.ctypes.data)
+        self.SetSolution(sol)
+        return sol
 
     @property
     def solution(self):

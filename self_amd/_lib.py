@@ -153,6 +153,8 @@ IO_SIGNATURES = {
     "selfamd_hopr_dataset_dims": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, c_void_p],
     "selfamd_hopr_read_int": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
     "selfamd_hopr_read_double": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
+    "selfamd_pickup_write": [ctypes.c_char_p, c_int, c_int, c_int] + [c_void_p] * 9,
+    "selfamd_pickup_read": [ctypes.c_char_p, c_int, c_int, c_int, c_void_p],
 }
 _io = None
 
@@ -172,6 +174,15 @@ def load_io():
         fn.restype = ctypes.c_char_p if name == "selfamd_io_last_error" else c_int
     _io = lib
     return lib
+
+
+def io_available():
+    """True when libself_amd_io.so is built and loads (its HDF5 library is present)."""
+    try:
+        load_io()
+        return True
+    except (ImportError, OSError):
+        return False
 
 
 def call_io(name, *args):

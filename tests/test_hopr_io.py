@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF = "/root/reference"
 FIXTURES = ["hopr_block2d", "hopr_geophysical_block2d_medium", "hopr_circle"]
 
-pytestmark = pytest.mark.skipif(not os.path.exists(_lib.IO_LIB_PATH), reason="libself_amd_io.so not built")
+pytestmark = pytest.mark.skipif(not _lib.io_available(), reason="libself_amd_io.so not built or HDF5 missing")
 
 
 def fixture(name):
@@ -32,7 +32,7 @@ def mesh_path(fx):
 
 
 def read(path, name, dtype):
-    rank, dims = ctypes.c_int32(), (ctypes.c_int64 * 2)()
+    rank, dims = ctypes.c_int32(), (ctypes.c_int64 * 5)()
     _lib.call_io("selfamd_hopr_dataset_dims", path, name.encode(), ctypes.byref(rank), dims)
     a = np.empty(tuple(dims[k] for k in range(rank.value)), dtype)
     fn = "selfamd_hopr_read_int" if dtype == np.int32 else "selfamd_hopr_read_double"
