@@ -14,6 +14,7 @@ PROGRAM dg2d_run
   USE SELF_Lagrange
   USE SELF_Mesh
   USE SELF_Model2D
+  USE SELF_IO
   IMPLICIT NONE
   INTEGER :: model = 2, N = 7, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
   LOGICAL :: periodic = .FALSE., nodal = .FALSE.
@@ -89,6 +90,11 @@ CONTAINS
     CALL m % ForwardStep(tn=REAL(nSteps, prec)*dt, dt=dt)
     CALL m % UpdateHost()
     CALL write_array(TRIM(prefix)//'_out.bin', m % solution % interior, SIZE(m % solution % interior))
+    ! pickup file of the final state, then read back into the model
+    CALL WriteModel(m, TRIM(prefix)//'_pickup.h5')
+    CALL ReadModel(m, TRIM(prefix)//'_pickup.h5')
+    CALL m % UpdateHost()
+    CALL write_array(TRIM(prefix)//'_reread.bin', m % solution % interior, SIZE(m % solution % interior))
     CALL m % Free()
   END SUBROUTINE run_model
 

@@ -43,3 +43,11 @@ def test_fortran_driver_parity(kind, model, N, n, periodic, bcid, tmp_path):
     assert abs(mass - c.om.mass(2)) <= TOL * abs(c.om.mass(2))
     c.om.rk3_steps(steps, dt)
     assert rel_err(out, c.om.solution).max() <= TOL
+    # WriteModel / ReadModel through SELF_IO: the pickup holds the final state exactly, with N
+    reread = np.fromfile(prefix + "_reread.bin").reshape(shape)
+    assert np.array_equal(reread, out)
+    from self_amd import _lib
+    if _lib.io_available():
+        sol = np.empty(shape)
+        _lib.call_io("selfamd_pickup_read", os.fsencode(prefix + "_pickup.h5"), N, shape[1], shape[0], sol.ctypes.data)
+        assert np.array_equal(sol, out)
