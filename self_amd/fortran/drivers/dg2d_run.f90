@@ -5,7 +5,9 @@
 !
 ! Usage: dg2d_run <parameter file>.  The namelist &run gives the model (1 LSW, 2 SW,
 ! 3 Euler), N, nx, ny, periodic, bcid, nSteps, dt, nodal (metric representation) and
-! prefix; <prefix>_sol.bin holds the initial interior(0:N,0:N,nVar,nElem) and
+! prefix, and optionally meshFile, a HOPr mesh file read by Read_HOPr in place of the
+! structured nx x ny mesh (bcid > 0 then resets its boundary conditions, as the reference's
+! programs do with ResetBoundaryConditionType); <prefix>_sol.bin holds the initial interior(0:N,0:N,nVar,nElem) and
 ! <prefix>_field1.bin / _field2.bin the static fields (0:N,0:N,nElem) in native doubles.
 ! Outputs: <prefix>_tend.bin (tendency of the initial state) and <prefix>_out.bin
 ! (solution after nSteps RK3 steps).
@@ -19,8 +21,8 @@ PROGRAM dg2d_run
   INTEGER :: model = 2, N = 7, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
   LOGICAL :: periodic = .FALSE., nodal = .FALSE.
   REAL(prec) :: dt = 1.0e-4_prec
-  CHARACTER(LEN=400) :: prefix = 'dg2d', parfile
-  NAMELIST /run/ model, N, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix
+  CHARACTER(LEN=400) :: prefix = 'dg2d', meshFile = '', parfile
+  NAMELIST /run/ model, N, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix, meshFile
   TYPE(Lagrange), TARGET :: interp
   TYPE(Mesh2D), TARGET :: mesh
   TYPE(SEMQuad), TARGET :: geometry
@@ -37,7 +39,12 @@ PROGRAM dg2d_run
   nVar = MERGE(4, 3, model == 3)
 
   CALL interp % Init(N, GAUSS, N, GAUSS)
-  CALL mesh % StructuredMesh(nx, ny, 0.0_prec, 1.0_prec, 0.0_prec, 1.0_prec, periodic, periodic, bcid)
+  IF (LEN_TRIM(meshFile) > 0) THEN
+    CALL Read_HOPr(mesh, TRIM(meshFile))
+    IF (bcid > 0) CALL mesh % ResetBoundaryConditionType(bcid)
+  ELSE
+    CALL mesh % StructuredMesh(nx, ny, 0.0_prec, 1.0_prec, 0.0_prec, 1.0_prec, periodic, periodic, bcid)
+  END IF
   CALL geometry % Init(interp, mesh % nElem)
   CALL geometry % GenerateFromMesh(mesh)
 

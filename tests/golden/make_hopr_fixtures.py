@@ -3,14 +3,17 @@
 Reads the reference's shipped HOPr mesh files (etc/mesh/*/..._mesh.h5, data the
 reference's own example programs load) with the h5dump tool and stores the datasets that
 Read_HOPr_Mesh2D consumes (src/SELF_Mesh.f90:852-1016) as JSON: ElemInfo, SideInfo,
-NodeCoords, GlobalNodeIDs, BCType and the Ngeo/nElems attributes.  Only this script reads
-the reference checkout, and only in the build container; the JSON files travel.
+NodeCoords, GlobalNodeIDs, BCType and the Ngeo/nElems attributes.  The Block2D mesh file
+itself (18 KB) is also copied unchanged to hopr_Block2D_mesh.h5, so that the HDF5 readers can
+be run where the reference checkout is absent.  Only this script reads the reference
+checkout, and only in the build container; the fixtures travel.
 
 Usage: python tests/golden/make_hopr_fixtures.py [/root/reference]
 """
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 
@@ -49,6 +52,7 @@ def main(ref):
         with open(os.path.join(here, key + ".json"), "w") as f:
             json.dump(fx, f, separators=(",", ":"))
         print(key, {k: (fx[k]["shape"] if isinstance(fx[k], dict) else fx[k]) for k in fx})
+    shutil.copyfile(os.path.join(ref, MESHES["hopr_block2d"]), os.path.join(here, "hopr_Block2D_mesh.h5"))
 
 
 if __name__ == "__main__":

@@ -51,3 +51,34 @@ def test_fortran_driver_parity(kind, model, N, n, periodic, bcid, tmp_path):
         sol = np.empty(shape)
         _lib.call_io("selfamd_pickup_read", os.fsencode(prefix + "_pickup.h5"), N, shape[1], shape[0], sol.ctypes.data)
         assert np.array_equal(sol, out)
+
+
+def test_fortran_driver_on_hopr_mesh(tmp_path):
+    """Read_HOPr of the Fortran SELF_IO module: the driver reads the Block2D HOPr mesh file
+    (committed copy of the reference's etc/mesh/Block2D/Block2D_mesh.h5), resets its boundary
+    conditions to no-normal-flow and runs SW at N = 5; tendency and RK3 steps must match the
+    oracle on the mesh that the Python reader builds from the same file."""
+    import self_amd
+    assert os.path.exists(DRIVER), "build the Fortran layer first (make -C self_amd/fortran)"
+    mesh_file = os.path.join(ROOT, "tests", "golden", "hopr_Block2D_mesh.h5")
+    mesh = self_amd.Mesh2D.read_hopr(mesh_file)
+    mesh.ResetBoundaryConditionType(self_amd.BC_NONORMALFLOW)
+    N = 5
+    c = Case("sw", N, 0, 0, hopr=mesh, seed=4, metrics="affine")
+    prefix = str(tmp_path / "hopr")
+    c.sol.astype(np.float64).tofile(prefix + "_sol.bin")
+    np.ascontiguousarray(c.fields["topo"]).tofile(prefix + "_field1.bin")
+    np.zeros_like(c.fields["topo"]).tofile(prefix + "_field2.bin")
+    dt, steps = c.dt(), 6
+    par = tmp_path / "run.nml"
+    par.write_text(f"&run model=2, N={N}, bcid={self_amd.BC_NONORMALFLOW}, nSteps={steps}, dt={float(dt)!r}, "
+                   f"nodal=.false., prefix='{prefix}', meshFile='{mesh_file}' /\n")
+    r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shape = c.sol.shape
+    assert shape[0] == 25
+    tend = np.fromfile(prefix + "_tend.bin").reshape(shape)
+    out = np.fromfile(prefix + "_out.bin").reshape(shape)
+    assert rel_err(tend, c.om.tendency()).max() <= TOL
+    c.om.rk3_steps(steps, dt)
+    assert rel_err(out, c.om.solution).max() <= TOL

@@ -1,7 +1,8 @@
 """HOPr mesh files read through libself_amd_io.so (HDF5 C library) against the committed
 fixtures, which tests/golden/make_hopr_fixtures.py wrote from the same files with h5dump
 (doubles printed with 17 digits, so the values compare bit-exactly).  Runs where the
-reference checkout with its mesh files is present (the build container)."""
+reference checkout with its mesh files is present (the build container); the Block2D cases
+also run elsewhere on the committed copy of that mesh file (tests/golden/hopr_Block2D_mesh.h5)."""
 import ctypes
 import json
 import os
@@ -24,8 +25,13 @@ def fixture(name):
         return json.load(f)
 
 
+BLOCK2D_COPY = os.path.join(HERE, "golden", "hopr_Block2D_mesh.h5")
+
+
 def mesh_path(fx):
     p = os.path.join(REF, fx["source"])
+    if not os.path.exists(p) and fx["source"] == "etc/mesh/Block2D/Block2D_mesh.h5":
+        p = BLOCK2D_COPY
     if not os.path.exists(p):
         pytest.skip(f"mesh file {fx['source']} not present (reference checkout absent)")
     return os.fsencode(p)
@@ -80,3 +86,13 @@ def test_errors_are_reported(tmp_path):
     buf = np.empty(3, np.int32)
     with pytest.raises(_lib.SelfAmdError, match="elements, buffer"):
         _lib.call_io("selfamd_hopr_read_int", p, b"SideInfo", buf.ctypes.data, buf.size)
+
+
+def test_committed_block2d_copy_matches_fixture():
+    """The committed copy of the Block2D mesh file reads as its JSON fixture (no reference needed)."""
+    fx = fixture("hopr_block2d")
+    p = os.fsencode(BLOCK2D_COPY)
+    for ds, dtype in (("SideInfo", np.int32), ("GlobalNodeIDs", np.int32), ("NodeCoords", np.float64)):
+        assert np.array_equal(read(p, ds, dtype).ravel(), np.array(fx[ds]["data"], dtype)), ds
+    m = self_amd.Mesh2D.read_hopr(BLOCK2D_COPY)
+    assert m.nElem == fx["nElems"] == 25 and m.nGeo == fx["Ngeo"] == 1
