@@ -49,3 +49,54 @@ def test_batch_rejects_mixed_kinds():
     _, m2, _ = _run("lsw", 3, 3, 1)
     with pytest.raises(self_amd.SelfAmdError):
         self_amd.Batch([m1, m2], [1e-3, 1e-3])
+
+
+def test_full_size_ensemble():
+    """The batched configuration as benchmarked: 512 SW runs, n x n periodic elements with
+    n ~ U{8..64} and N ~ U{3,5,7} (seed 0).  After 5 batched steps, 16 runs spread over the
+    batch equal the same runs advanced alone bit for bit, and two of them (one per end of the
+    degree range) match the oracle."""
+    import oracle
+    from tests.cases import KIND, flatten_metrics
+    rng = np.random.default_rng(0)
+    n_runs, steps = 512, 5
+    sizes = rng.integers(8, 65, size=n_runs)
+    degrees = rng.choice([3, 5, 7], size=n_runs)
+    cache, models, dts, inits = {}, [], [], []
+    for r in range(n_runs):
+        n, N = int(sizes[r]), int(degrees[r])
+        if (n, N) not in cache:
+            mesh = self_amd.Mesh2D.structured(n, n, periodic=(True, True))
+            interp = self_amd.Lagrange(N, self_amd.GAUSS, N, self_amd.GAUSS)
+            geom = self_amd.SEMQuad(interp, mesh)
+            cache[(n, N)] = (mesh, interp, geom, geom.x.to_host(), geom.CovariantArcMin())
+        mesh, interp, geom, x, arcmin = cache[(n, N)]
+        sol, topo = workloads.nonlinear_sw(x, interp.controlPoints, interp.qWeights, seed=1000 + r)
+        m = self_amd.ShallowWater(mesh, geom)
+        m.SetTopography(topo)
+        m.SetSolution(sol)
+        models.append(m)
+        dts.append(workloads.cfl_dt(arcmin, 1.2, 0.5))
+        inits.append((sol, topo))
+    b = self_amd.Batch(models, dts)
+    groups, _ = b.info()
+    assert groups == 3
+    b.ForwardStep(steps)
+    b.scatter()
+    for r in np.linspace(0, n_runs - 1, 16).astype(int):
+        mesh, interp, geom, _, _ = cache[(int(sizes[r]), int(degrees[r]))]
+        m = self_amd.ShallowWater(mesh, geom)
+        m.SetTopography(inits[r][1])
+        m.SetSolution(inits[r][0])
+        m.ForwardStep(steps, dts[r])
+        np.testing.assert_array_equal(models[r].solution, m.solution, err_msg=f"run {r}")
+    for N in (3, 7):
+        r = int(np.flatnonzero(degrees == N)[0])
+        mesh = cache[(int(sizes[r]), N)][0]
+        og = oracle.Geometry(N, mesh.nodeCoords, mesh.nGeo)
+        flatten_metrics(og)
+        om = oracle.Model(KIND["sw"], N, mesh.sideInfo, og)
+        om.set_topography(inits[r][1])
+        om.set_solution(inits[r][0])
+        om.rk3_steps(steps, dts[r])
+        assert rel_err(models[r].solution, om.solution).max() <= 1e-12, f"run {r}"

@@ -159,6 +159,24 @@ def test_full_size_sw_256():
     assert abs(mass(s) - m0) <= 1e-13 * m0
 
 
+def test_lsw_10x10_full_run():
+    """The linear shallow-water configuration as benchmarked (10^2 elements, N = 7, reflecting
+    walls, the gravity-wave state, H = 1, no Coriolis, CFL 0.5): all 200 RK3 steps against the
+    oracle, and the energy (CalculateEntropy) does not grow: the Rusanov flux only dissipates
+    and the walls let nothing in."""
+    c = Case("lsw", 7, 10, 10, periodic=False, bcid=self_amd.BC_NONORMALFLOW, seed=0, metrics="affine",
+             generic=False)
+    m, _ = c.gpu_model()
+    dt = c.dt()
+    e0 = m.CalculateEntropy()
+    m.ForwardStep(200, dt)
+    c.om.rk3_steps(200, dt)
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+    e1 = m.CalculateEntropy()
+    assert e1 <= e0 * (1 + 1e-14), (e0, e1)
+    assert e1 < e0
+
+
 def test_full_size_euler_512():
     """Euler configuration at its benchmarked size (512^2 elements, N = 7, no-normal-flow
     walls, gravity potential, LLF, affine metrics): one RK3 step against the oracle run with
