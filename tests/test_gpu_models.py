@@ -53,6 +53,35 @@ def test_rk3_trajectory_parity(kind, N, n, periodic, bcid, metrics):
 
 
 @pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
+def test_every_degree(kind, N):
+    """Every degree the engine instantiates (N = 1..7), each physics, both metric modes: the
+    tendency and three RK3 steps on a 3 x 3 walled mesh against the oracle."""
+    for metrics in ("nodal", "affine"):
+        c = Case(kind, N, 3, 3, periodic=False, bcid=self_amd.BC_NONORMALFLOW, seed=12 + N, metrics=metrics)
+        m, _ = c.gpu_model()
+        assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL, metrics
+        dt = c.dt(cfl=0.5 if N >= 3 else 0.05)
+        m.ForwardStep(3, dt)
+        c.om.rk3_steps(3, dt)
+        assert rel_err(m.solution, c.om.solution).max() <= TOL, metrics
+
+
+@pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
+@pytest.mark.parametrize("n,periodic", [(1, False), (2, True)])
+def test_smallest_meshes(kind, n, periodic):
+    """The smallest meshes: one walled element (every side a boundary), and 2 x 2 periodic,
+    where each element meets the same neighbour across two opposite sides.  (The structured
+    generator needs n >= 2 for a periodic direction.)"""
+    c = Case(kind, 5, n, n, periodic=periodic, bcid=self_amd.BC_NONORMALFLOW, seed=21, metrics="affine")
+    m, _ = c.gpu_model()
+    assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
+    m.ForwardStep(4, c.dt())
+    c.om.rk3_steps(4, c.dt())
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+
+
+@pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
 def test_end_to_end_device_geometry(kind):
     """Mesh -> device-generated metrics -> nodal-metric engine, against the oracle's own
     geometry: the whole reference pipeline on identical mesh and state inputs.  The device
