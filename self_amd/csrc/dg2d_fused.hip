@@ -50,9 +50,12 @@ struct FusedArgs {
   double *S, *G, *dSdt;
   const double *traceIn;
   double *traceOut;
-  // SW lagged normal velocity u.n of each face node, own side only, (0:N,4,nEl): read from
-  // the previous stage's buffer (the neighbour's value, negated, is the outer side) and
-  // written to the other buffer for the next stage
+  // SW lagged velocity traces, read from the previous stage's buffer and written to the other
+  // one for the next stage.  Affine mode: u.n of each face node, own side only, (0:N,4,nEl);
+  // the outer side's value is the neighbour's own, negated (the two sides' normals are
+  // opposite).  Nodal mode: the vector (u, v) of each face node, (2,0:N,4,nEl), and both normal
+  // velocities are formed with this side's normal, as the reference does, because on curved
+  // elements interpolated below the mesh degree the two sides' normals differ.
   const double *unIn;
   double *unOut;
   const double *aux;        // SW topography / LSW bathymetry / Euler potential, or null
@@ -249,6 +252,19 @@ __device__ inline void phys_nodal(const double *s, const Params &p, double *q) {
 // Sum of x over the lane pair l, l ^ 32 of a wavefront, returned to both lanes.  The
 // exchange is v_permlane32_swap (gfx950), a VALU operation: it does not use the LDS pipe that
 // a shuffle through ds_bpermute would occupy.  Both lanes get the same sum x_l + x_(l+32).
+// Lagged SW normal velocities (the layouts are described with FusedArgs::unIn): the own side's
+// and the outer side's u.n at face-node slot fb (the neighbour's slot for the outer side).
+template <bool NODAL>
+__device__ inline double lag_own(const double *un, size_t fb, double nx, double ny) {
+  if constexpr (NODAL) return un[2 * fb] * nx + un[2 * fb + 1] * ny;
+  return un[fb];
+}
+template <bool NODAL>
+__device__ inline double lag_outer(const double *un, size_t fbNeighbour, double nx, double ny) {
+  if constexpr (NODAL) return un[2 * fbNeighbour] * nx + un[2 * fbNeighbour + 1] * ny;
+  return -un[fbNeighbour];
+}
+
 __device__ inline double pair_sum32(double x) {
   const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
   const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
@@ -387,6 +403,12 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   int code = 0;
   if (faceActive && fhalf == 0) {
     code = a.nbr[4 * (size_t)fe + side];
+    // normals: first in nodal mode, where the lagged normal velocities need them; after the
+    // neighbour loads in affine mode (measured slightly faster there)
+    if constexpr (NODAL) {
+      const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
+      nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
+    }
     if (code >= 0) {
       const int flip = code & 1, s2 = (code >> 1) & 3;
       const size_t e2 = (size_t)(code >> 3);
@@ -394,13 +416,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       const double *tr = a.traceIn + (size_t)(flip ? N - fi : fi) + NP * (size_t)NT * (s2 + 4 * e2);
 #pragma unroll
       for (int k = 0; k < NT; k++) qR[k] = tr[(size_t)k * NP];
-      if (Phys::kLag) unR = -a.unIn[nb];
+      if (Phys::kLag) unR = lag_outer<NODAL>(a.unIn, nb, nx, ny);
     }
-    if (Phys::kLag) unL = a.unIn[(size_t)fi + NP * (side + 4 * (size_t)fe)];
-    if constexpr (NODAL) {
-      const size_t fb = (size_t)fi + NP * (side + 4 * (size_t)fe);
-      nx = a.nnHat[2 * fb]; ny = a.nnHat[2 * fb + 1]; nmag = a.nnScale[fb];
-    } else {
+    if (Phys::kLag) unL = lag_own<NODAL>(a.unIn, (size_t)fi + NP * (side + 4 * (size_t)fe), nx, ny);
+    if constexpr (!NODAL) {
       const double *gr = a.geom + (size_t)fe * kGeomStride;
       nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
     }
@@ -485,10 +504,15 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     }
     if (Phys::kLag) {
       // velocity traces lag one stage (src/models/SELF_ShallowWater.f90:222-237): the stored
-      // normal velocities are used now and this stage's are stored for the next one.  The
-      // outer side's u.n is the neighbour's own value with the opposite normal.
+      // velocities are used now and this stage's are stored for the next one
       const double invH = 1.0 / qL[2];
-      a.unOut[(size_t)fi_ + NP * (side_ + 4 * (size_t)fe_)] = (qL[0] * invH) * nx_ + (qL[1] * invH) * ny_;
+      const size_t fb = (size_t)fi_ + NP * (side_ + 4 * (size_t)fe_);
+      if constexpr (NODAL) {
+        a.unOut[2 * fb] = qL[0] * invH;
+        a.unOut[2 * fb + 1] = qL[1] * invH;
+      } else {
+        a.unOut[fb] = (qL[0] * invH) * nx_ + (qL[1] * invH) * ny_;
+      }
     }
     double fstar[NV];
     Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, unR_, a.p, fstar);
@@ -506,14 +530,6 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       const int fe2 = a.elems ? a.elems[pf2] : pf2;
       double qR2[NT], unL2 = 0.0, unR2 = 0.0, nx2, ny2, nmag2;
       const int code2 = a.nbr[4 * (size_t)fe2 + side2];
-      if (code2 >= 0) {
-        const int flip = code2 & 1, s2 = (code2 >> 1) & 3;
-        const size_t e2 = (size_t)(code2 >> 3);
-        const double *tr = a.traceIn + (size_t)(flip ? N - fi2 : fi2) + NP * (size_t)NT * (s2 + 4 * e2);
-        for (int k = 0; k < NT; k++) qR2[k] = tr[(size_t)k * NP];
-        if (Phys::kLag) unR2 = -a.unIn[(size_t)(flip ? N - fi2 : fi2) + NP * (s2 + 4 * e2)];
-      }
-      if (Phys::kLag) unL2 = a.unIn[(size_t)fi2 + NP * (side2 + 4 * (size_t)fe2)];
       if constexpr (NODAL) {
         const size_t fb = (size_t)fi2 + NP * (side2 + 4 * (size_t)fe2);
         nx2 = a.nnHat[2 * fb]; ny2 = a.nnHat[2 * fb + 1]; nmag2 = a.nnScale[fb];
@@ -521,6 +537,14 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
         const double *gr = a.geom + (size_t)fe2 * kGeomStride;
         nx2 = gr[8 + side2]; ny2 = gr[12 + side2]; nmag2 = gr[16 + side2];
       }
+      if (code2 >= 0) {
+        const int flip = code2 & 1, s2 = (code2 >> 1) & 3;
+        const size_t e2 = (size_t)(code2 >> 3);
+        const double *tr = a.traceIn + (size_t)(flip ? N - fi2 : fi2) + NP * (size_t)NT * (s2 + 4 * e2);
+        for (int k = 0; k < NT; k++) qR2[k] = tr[(size_t)k * NP];
+        if (Phys::kLag) unR2 = lag_outer<NODAL>(a.unIn, (size_t)(flip ? N - fi2 : fi2) + NP * (s2 + 4 * e2), nx2, ny2);
+      }
+      if (Phys::kLag) unL2 = lag_own<NODAL>(a.unIn, (size_t)fi2 + NP * (side2 + 4 * (size_t)fe2), nx2, ny2);
       face(fel2, side2, fi2, fe2, fnode2, code2, qR2, unL2, unR2, nx2, ny2, nmag2);
     }
   }
@@ -758,7 +782,12 @@ __global__ void k_sw_un_init(const double *trace, double *un, const double *geom
       nx = gr[8 + side]; ny = gr[12 + side];
     }
     const double *tl = trace + fi + NP * 3 * (side + 4 * e);
-    un[t] = (tl[0] / tl[2 * NP]) * nx + (tl[NP] / tl[2 * NP]) * ny;
+    if (nnHat) {
+      un[2 * t] = tl[0] / tl[2 * NP];
+      un[2 * t + 1] = tl[NP] / tl[2 * NP];
+    } else {
+      un[t] = (tl[0] / tl[2 * NP]) * nx + (tl[NP] / tl[2 * NP]) * ny;
+    }
   }
 }
 
@@ -1123,8 +1152,9 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
     set_error("model2d_create: device allocation failed (%zu doubles of state)", nS);
     return fail(kHipError);
   }
-  if (kind == SELFAMD_MODEL_SW && (hipMalloc(&m->un[0], sizeof(double) * (size_t)P * 4 * nEl) != hipSuccess ||
-                                   hipMalloc(&m->un[1], sizeof(double) * (size_t)P * 4 * nEl) != hipSuccess)) {
+  // lagged velocities: room for the nodal mode's vectors (2 doubles per face node)
+  if (kind == SELFAMD_MODEL_SW && (hipMalloc(&m->un[0], sizeof(double) * 2 * (size_t)P * 4 * nEl) != hipSuccess ||
+                                   hipMalloc(&m->un[1], sizeof(double) * 2 * (size_t)P * 4 * nEl) != hipSuccess)) {
     set_error("model2d_create: device allocation failed (un)");
     return fail(kHipError);
   }
@@ -1139,8 +1169,8 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
       hipMemsetAsync(m->G, 0, sizeof(double) * nS, m->stream) != hipSuccess ||
       hipMemsetAsync(m->trace[0], 0, sizeof(double) * nT, m->stream) != hipSuccess ||
       hipMemsetAsync(m->trace[1], 0, sizeof(double) * nT, m->stream) != hipSuccess ||
-      (m->un[0] && hipMemsetAsync(m->un[0], 0, sizeof(double) * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
-      (m->un[1] && hipMemsetAsync(m->un[1], 0, sizeof(double) * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
+      (m->un[0] && hipMemsetAsync(m->un[0], 0, sizeof(double) * 2 * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
+      (m->un[1] && hipMemsetAsync(m->un[1], 0, sizeof(double) * 2 * (size_t)P * 4 * nEl, m->stream) != hipSuccess) ||
       hipStreamSynchronize(m->stream) != hipSuccess) {
     set_error("model2d_create: device initialisation failed");
     return fail(kHipError);
@@ -1175,6 +1205,14 @@ int selfamd_model2d_set_param(selfamd_model2d *m, int which, double value) {
     case SELFAMD_PARAM_RGAS: m->p.Rgas = value; break;
     case SELFAMD_PARAM_NODAL_METRICS:
       SELFAMD_REQUIRE(value != 0.0 || m->affine, "model2d_set_param: the mesh is not affine; nodal metrics required");
+      if (m->nodal != (value != 0.0) && m->kind == SELFAMD_MODEL_SW) {
+        // the two modes keep the SW velocity lag in different layouts: restart it from the
+        // current traces, as SetSolution does
+        m->nodal = value != 0.0;
+        SELFAMD_HIP(launch_un_init(m->N, m->trace[m->cur], m->un[m->cur], m->geom, m->nodal ? m->nnHat : nullptr,
+                                   m->nEl, m->stream));
+        SELFAMD_HIP(hipStreamSynchronize(m->stream));
+      }
       m->nodal = value != 0.0;
       break;
     default: SELFAMD_REQUIRE(false, "model2d_set_param: unknown parameter %d", which);
@@ -1282,7 +1320,8 @@ int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {
   SELFAMD_HIP(m->launch_stage(1, 0, 0.0, m->cur, dSdt));
   if (m->un[0]) {
     // the evaluation's velocities become the lag state of the next stage
-    SELFAMD_HIP(hipMemcpyAsync(m->un[m->cur], m->un[m->cur ^ 1], sizeof(double) * (m->N + 1) * 4 * (size_t)m->nEl,
+    SELFAMD_HIP(hipMemcpyAsync(m->un[m->cur], m->un[m->cur ^ 1],
+                               sizeof(double) * (m->nodal ? 2 : 1) * (m->N + 1) * 4 * (size_t)m->nEl,
                                hipMemcpyDeviceToDevice, m->stream));
   }
   SELFAMD_HIP(hipStreamSynchronize(m->stream));

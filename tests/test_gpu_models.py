@@ -162,6 +162,31 @@ def test_hopr_block_mesh_parity():
     assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
+@pytest.mark.parametrize("kind", ["sw", "lsw", "euler"])
+def test_curved_circle_mesh(kind):
+    """The reference's curved Circle mesh (HOPr, Ngeo = 6, 56 elements) with no-normal-flow
+    walls: the engine rejects the affine representation there and, with per-node metrics,
+    matches the oracle on the tendency and four RK3 steps."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hopr_circle.json")) as f:
+        fx = json.load(f)
+    mesh = self_amd.Mesh2D.from_hopr(
+        fx["Ngeo"], np.array(fx["SideInfo"]["data"], np.int32).reshape(fx["SideInfo"]["shape"]),
+        np.array(fx["NodeCoords"]["data"]).reshape(fx["NodeCoords"]["shape"]),
+        np.array(fx["GlobalNodeIDs"]["data"], np.int32))
+    mesh.ResetBoundaryConditionType(self_amd.BC_NONORMALFLOW)
+    c = Case(kind, 5, 0, 0, hopr=mesh, seed=5, metrics="nodal")
+    m, _ = c.gpu_model()
+    with pytest.raises(self_amd.SelfAmdError):
+        m.SetNodalMetrics(False)
+    m.SetNodalMetrics(True)
+    assert rel_err(m.CalculateTendency(), c.om.tendency()).max() <= TOL
+    m.ForwardStep(4, c.dt())
+    c.om.rk3_steps(4, c.dt())
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+
+
 def test_full_size_sw_256():
     """Headline configuration (256^2 elements, N = 7, periodic, affine metrics as benchmarked):
     two RK3 steps against the oracle run with all host threads, then mass conservation over
