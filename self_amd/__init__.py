@@ -93,6 +93,33 @@ def device_name(device=0):
     return buf.value.decode()
 
 
+def time_stamp(t):
+    """13-digit time stamp dddd hh mm ss mmm of t seconds (TimeStamp with units 's',
+    src/SELF_SupportRoutines.f90:856-924): day, hour, minute, second, millisecond."""
+    day = int(t / 86400.0)
+    hour = int((t - 86400.0 * day) / 3600.0)
+    minute = int((t - 3600.0 * hour - 86400.0 * day) / 60.0)
+    rest = t - 60.0 * minute - 3600.0 * hour - 86400.0 * day
+    second = int(rest)
+    ms = int(np.floor((rest - second) * 1000.0 + 0.5))  # nearest, halves away from zero
+    if ms >= 1000:
+        ms, second = ms - 1000, second + 1
+    if second >= 60:
+        second, minute = second - 60, minute + 1
+    if minute >= 60:
+        minute, hour = minute - 60, hour + 1
+    if hour >= 24:
+        hour, day = hour - 24, day + 1
+    return f"{day:04d}{hour:02d}{minute:02d}{second:02d}{ms:03d}"
+
+
+def _es16(v):
+    """v in the Fortran edit descriptor ES16.7E3: one digit before the point, seven after,
+    a three-digit exponent, right-justified in 16 columns."""
+    m, e = f"{v:.7E}".split("E")
+    return f"{m}E{int(e):+04d}".rjust(16)
+
+
 def synchronize():
     call("selfamd_device_synchronize")
 
@@ -340,6 +367,42 @@ class Model2D:
         call_io("selfamd_pickup_write", os.fsencode(os.path.abspath(path)), self.N, self.nVar, self.nElem,
                 *[a.ctypes.data for a in host])
 
+    def WriteTecplot(self, filename=None):
+        """WriteTecplot (WriteTecplot_Model2D, src/SELF_Model2D.f90:1101-1207): the element
+        positions and the solution interpolated on the device to the interpolant's target grid
+        ((M+1)^2 points per element, GridInterp), written as ASCII Tecplot, one zone per
+        element, X, Y, the variables, then d/dx and d/dy of each variable.  The gradient
+        columns are zero: no tendency on this path forms the solution gradient (the
+        reference's 2-D models allocate that register and never fill it).  Default file name:
+        solution.<time stamp>.tec.  Returns the file name."""
+        interp, g = self.geometry.interp, self.geometry
+        N, M, nEl, nVar = self.N, interp.M, self.nElem, self.nVar
+        if filename is None:
+            filename = f"solution.{time_stamp(self.t)}.tec"
+        iM = DeviceArray.from_host(np.ascontiguousarray(interp.iMatrix))
+        s = DeviceArray(self.shape)
+        call("selfamd_model2d_get_solution", self._h, _ptr(s))
+        sT = DeviceArray((nEl, nVar, M + 1, M + 1))
+        xT = DeviceArray((nEl, M + 1, M + 1, 2))
+        call("selfamd_scalar_grid_interp_2d", iM.ptr, s.ptr, sT.ptr, N, M, nVar, nEl, None)
+        call("selfamd_vector_grid_interp_2d", iM.ptr, g.x.ptr, xT.ptr, N, M, 1, nEl, None)
+        sol, x = sT.to_host(), xT.to_host()
+        names = self.var_names
+        header = 'VARIABLES = "X", "Y"' + "".join(f', "{n}"' for n in names) + "".join(
+            f', "d/dx({n})"' for n in names) + "".join(f', "d/dy({n})"' for n in names)
+        # the edit descriptors (ES16.7E3, 1X) repeated: one blank between values and none at the
+        # end of a record (a trailing X edit writes nothing)
+        zeros = "".join(" " + _es16(0.0) for _ in range(2 * nVar))
+        with open(filename, "w") as f:
+            f.write(" " + header + "\n")
+            for e in range(nEl):
+                f.write(f' ZONE T="el{e + 1:08d}", I={M + 1}, J={M + 1}\n')
+                for j in range(M + 1):
+                    for i in range(M + 1):
+                        vals = [x[e, j, i, 0], x[e, j, i, 1]] + [sol[e, v, j, i] for v in range(nVar)]
+                        f.write(" ".join(_es16(v) for v in vals) + zeros + "\n")
+        return filename
+
     def ReadModel(self, path):
         """ReadModel (Read_Model2D, src/SELF_Model2D.f90:1061-1099): the solution of a pickup
         file (its N must match), set with SetSolution.  The SW velocity lag restarts from the
@@ -418,6 +481,7 @@ class Model2D:
 class ShallowWater(Model2D):
     """Nonlinear shallow water (src/models/SELF_ShallowWater.f90): state (Hu, Hv, H)."""
     kind = MODEL_SW
+    var_names = ("Hu", "Hv", "H")  # src/models/SELF_ShallowWater.f90:135-143
 
     def __init__(self, mesh, geometry, g=1.0, fCori=0.0):
         super().__init__(mesh, geometry)
@@ -431,6 +495,7 @@ class ShallowWater(Model2D):
 class LinearShallowWater(Model2D):
     """Linear shallow water (src/models/SELF_LinearShallowWater.f90): state (u, v, eta)."""
     kind = MODEL_LSW
+    var_names = ("u", "v", "eta")  # src/models/SELF_LinearShallowWater.f90:139-147
 
     def __init__(self, mesh, geometry, g=1.0):
         super().__init__(mesh, geometry)
@@ -452,6 +517,7 @@ class CompressibleIdealGas2D(Model2D):
     (rho u, rho v, rho, rho E).  Boundary sides with bcid 100/101 use the prescribed state
     taken from the last SetSolution (SetPrescribedSolutionFromSolution)."""
     kind = MODEL_EULER
+    var_names = ("rho*u", "rho*v", "rho", "rho*E")  # src/models/SELF_CompressibleIdealGas2D.f90:306-318
 
     def __init__(self, mesh, geometry, gamma=1.4, R=1.0):
         super().__init__(mesh, geometry)

@@ -42,6 +42,7 @@ MODULE SELF_Model2D
     PROCEDURE, PUBLIC :: ReportEntropy => ReportEntropy_Model2D
     PROCEDURE, PUBLIC :: Integral => Integral_Model2D
     PROCEDURE, PUBLIC :: SetNodalMetrics => SetNodalMetrics_Model2D
+    PROCEDURE, PUBLIC :: WriteTecplot => WriteTecplot_Model2D
     PROCEDURE, PRIVATE :: SetField => SetField_Model2D
   END TYPE Model2D
 
@@ -194,6 +195,123 @@ CONTAINS
     WRITE (entropy, "(ES16.7E3)") this % entropy
     WRITE (*, '(A)') 't_i =' // TRIM(modelTime) // '  |  e_i =' // TRIM(entropy)
   END SUBROUTINE ReportEntropy_Model2D
+
+  ! WriteTecplot_Model2D (src/SELF_Model2D.f90:1101-1207): element positions and solution
+  ! interpolated on the device to the interpolant's target grid ((M+1)^2 points per element,
+  ! GridInterp), ASCII Tecplot, one zone per element: X, Y, the variables, then d/dx and d/dy
+  ! of each variable.  The gradient columns are zero: no tendency on this path forms the
+  ! solution gradient (the reference's 2-D models allocate that register and never fill it).
+  ! Default file name solution.<TimeStamp(t)>.tec.
+  SUBROUTINE WriteTecplot_Model2D(this, filename)
+    CLASS(Model2D), INTENT(inout) :: this
+    CHARACTER(*), INTENT(in), OPTIONAL :: filename
+    TYPE(Lagrange), POINTER :: interp
+    REAL(prec), ALLOCATABLE, TARGET :: xT(:,:,:,:), sT(:,:,:,:)
+    REAL(prec), ALLOCATABLE :: zeros(:)
+    TYPE(C_PTR) :: xT_dev, sT_dev
+    CHARACTER(LEN=512) :: tecFile
+    CHARACTER(LEN=1024) :: header
+    CHARACTER(LEN=32) :: fmat
+    INTEGER :: N, M, nVar, nEl, iEl, i, j, k, u
+    interp => this % solution % interp
+    N = interp % N
+    M = interp % M
+    nVar = this % solution % nVar
+    nEl = this % solution % nElem
+    IF (PRESENT(filename)) THEN
+      tecFile = filename
+    ELSE
+      tecFile = 'solution.'//TimeStamp(this % t)//'.tec'
+    END IF
+    CALL selfamd_check(selfamd_model2d_get_solution(this % engine, this % solution % interior_dev), &
+                       'Model2D%WriteTecplot')
+    xT_dev = device_alloc(INT(2*(M + 1)**2*nEl, C_SIZE_T)*8_C_SIZE_T)
+    sT_dev = device_alloc(INT(nVar*(M + 1)**2*nEl, C_SIZE_T)*8_C_SIZE_T)
+    CALL selfamd_check(selfamd_vector_grid_interp_2d(interp % iMatrix_dev, this % geometry % x, xT_dev, INT(N, C_INT), &
+                       INT(M, C_INT), 1_C_INT, INT(nEl, C_INT), C_NULL_PTR), 'Model2D%WriteTecplot')
+    CALL selfamd_check(selfamd_scalar_grid_interp_2d(interp % iMatrix_dev, this % solution % interior_dev, sT_dev, &
+                       INT(N, C_INT), INT(M, C_INT), INT(nVar, C_INT), INT(nEl, C_INT), C_NULL_PTR), &
+                       'Model2D%WriteTecplot')
+    ALLOCATE (xT(2, 0:M, 0:M, nEl), sT(0:M, 0:M, nVar, nEl), zeros(2*nVar))
+    zeros = 0.0_prec
+    CALL hip_check(selfamd_memcpy(C_LOC(xT), xT_dev, INT(SIZE(xT), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
+                   'selfamd_memcpy')
+    CALL hip_check(selfamd_memcpy(C_LOC(sT), sT_dev, INT(SIZE(sT), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
+                   'selfamd_memcpy')
+    CALL device_free(xT_dev)
+    CALL device_free(sT_dev)
+    header = 'VARIABLES = "X", "Y"'
+    DO k = 1, nVar
+      header = TRIM(header)//', "'//TRIM(VarName(this % kind, k))//'"'
+    END DO
+    DO k = 1, nVar
+      header = TRIM(header)//', "d/dx('//TRIM(VarName(this % kind, k))//')"'
+    END DO
+    DO k = 1, nVar
+      header = TRIM(header)//', "d/dy('//TRIM(VarName(this % kind, k))//')"'
+    END DO
+    WRITE (fmat, '(A,I0,A)') '(', 3*nVar + 2, '(ES16.7E3,1X))'
+    OPEN (NEWUNIT=u, FILE=TRIM(tecFile), FORM='formatted', STATUS='replace', ACTION='write')
+    WRITE (u, '(A)') ' '//TRIM(header)
+    DO iEl = 1, nEl
+      WRITE (u, '(A,I8.8,A,I0,A,I0)') ' ZONE T="el', iEl, '", I=', M + 1, ', J=', M + 1
+      DO j = 0, M
+        DO i = 0, M
+          WRITE (u, fmat) xT(1, i, j, iEl), xT(2, i, j, iEl), sT(i, j, 1:nVar, iEl), zeros
+        END DO
+      END DO
+    END DO
+    CLOSE (u)
+  END SUBROUTINE WriteTecplot_Model2D
+
+  ! solution variable names (SetName in the models' Init: src/models/SELF_ShallowWater.f90:135-143,
+  ! SELF_LinearShallowWater.f90:139-147, SELF_CompressibleIdealGas2D.f90:306-318)
+  FUNCTION VarName(kind, k) RESULT(name)
+    INTEGER, INTENT(in) :: kind, k
+    CHARACTER(LEN=8) :: name
+    CHARACTER(LEN=8), PARAMETER :: sw(3) = [CHARACTER(LEN=8) :: 'Hu', 'Hv', 'H']
+    CHARACTER(LEN=8), PARAMETER :: lsw(3) = [CHARACTER(LEN=8) :: 'u', 'v', 'eta']
+    CHARACTER(LEN=8), PARAMETER :: eu(4) = [CHARACTER(LEN=8) :: 'rho*u', 'rho*v', 'rho', 'rho*E']
+    SELECT CASE (kind)
+    CASE (SELF_MODEL_SW)
+      name = sw(k)
+    CASE (SELF_MODEL_LSW)
+      name = lsw(k)
+    CASE DEFAULT
+      name = eu(k)
+    END SELECT
+  END FUNCTION VarName
+
+  ! TimeStamp with units 's' (src/SELF_SupportRoutines.f90:856-924): dddd hh mm ss mmm
+  FUNCTION TimeStamp(t) RESULT(stamp)
+    REAL(prec), INTENT(in) :: t
+    CHARACTER(LEN=13) :: stamp
+    INTEGER :: day, hour, minute, second, ms
+    REAL(prec) :: rest
+    day = INT(t/86400.0_prec)
+    hour = INT((t - 86400.0_prec*day)/3600.0_prec)
+    minute = INT((t - 3600.0_prec*hour - 86400.0_prec*day)/60.0_prec)
+    rest = t - 60.0_prec*minute - 3600.0_prec*hour - 86400.0_prec*day
+    second = INT(rest)
+    ms = NINT((rest - second)*1000.0_prec)
+    IF (ms >= 1000) THEN
+      ms = ms - 1000
+      second = second + 1
+    END IF
+    IF (second >= 60) THEN
+      second = second - 60
+      minute = minute + 1
+    END IF
+    IF (minute >= 60) THEN
+      minute = minute - 60
+      hour = hour + 1
+    END IF
+    IF (hour >= 24) THEN
+      hour = hour - 24
+      day = day + 1
+    END IF
+    WRITE (stamp, '(I4.4,I2.2,I2.2,I2.2,I3.3)') day, hour, minute, second, ms
+  END FUNCTION TimeStamp
 
   ! sum of w_i w_j J S(i,j,var) over the mesh (var counted from 1, as in the reference)
   REAL(prec) FUNCTION Integral_Model2D(this, var) RESULT(v)

@@ -10,7 +10,8 @@
 ! programs do with ResetBoundaryConditionType); <prefix>_sol.bin holds the initial interior(0:N,0:N,nVar,nElem) and
 ! <prefix>_field1.bin / _field2.bin the static fields (0:N,0:N,nElem) in native doubles.
 ! Outputs: <prefix>_tend.bin (tendency of the initial state) and <prefix>_out.bin
-! (solution after nSteps RK3 steps).
+! (solution after nSteps RK3 steps).  With Mtarget >= 1 in the namelist the interpolant's target
+! grid is Mtarget+1 uniform points and the final state is also written by WriteTecplot to <prefix>.tec.
 PROGRAM dg2d_run
   USE SELF_Constants
   USE SELF_Lagrange
@@ -18,11 +19,11 @@ PROGRAM dg2d_run
   USE SELF_Model2D
   USE SELF_IO
   IMPLICIT NONE
-  INTEGER :: model = 2, N = 7, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
+  INTEGER :: model = 2, N = 7, Mtarget = -1, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
   LOGICAL :: periodic = .FALSE., nodal = .FALSE.
   REAL(prec) :: dt = 1.0e-4_prec
   CHARACTER(LEN=400) :: prefix = 'dg2d', meshFile = '', parfile
-  NAMELIST /run/ model, N, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix, meshFile
+  NAMELIST /run/ model, N, Mtarget, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix, meshFile
   TYPE(Lagrange), TARGET :: interp
   TYPE(Mesh2D), TARGET :: mesh
   TYPE(SEMQuad), TARGET :: geometry
@@ -38,7 +39,11 @@ PROGRAM dg2d_run
   CLOSE (u)
   nVar = MERGE(4, 3, model == 3)
 
-  CALL interp % Init(N, GAUSS, N, GAUSS)
+  IF (Mtarget >= 1) THEN
+    CALL interp % Init(N, GAUSS, Mtarget, UNIFORM)
+  ELSE
+    CALL interp % Init(N, GAUSS, N, GAUSS)
+  END IF
   IF (LEN_TRIM(meshFile) > 0) THEN
     CALL Read_HOPr(mesh, TRIM(meshFile))
     IF (bcid > 0) CALL mesh % ResetBoundaryConditionType(bcid)
@@ -97,6 +102,7 @@ CONTAINS
     CALL m % ForwardStep(tn=REAL(nSteps, prec)*dt, dt=dt)
     CALL m % UpdateHost()
     CALL write_array(TRIM(prefix)//'_out.bin', m % solution % interior, SIZE(m % solution % interior))
+    IF (Mtarget >= 1) CALL m % WriteTecplot(TRIM(prefix)//'.tec')
     ! pickup file of the final state, then read back into the model
     CALL WriteModel(m, TRIM(prefix)//'_pickup.h5')
     CALL ReadModel(m, TRIM(prefix)//'_pickup.h5')

@@ -97,6 +97,12 @@ MODULE SELF_AMD_CAPI
       TYPE(C_PTR), VALUE :: iM, f, fNew, stream
       INTEGER(C_INT), VALUE :: N, M, nVar, nEl
     END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_vector_grid_interp_2d(iM, f, fNew, N, M, nVar, nEl, stream) &
+        BIND(C, name="selfamd_vector_grid_interp_2d")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: iM, f, fNew, stream
+      INTEGER(C_INT), VALUE :: N, M, nVar, nEl
+    END FUNCTION
     INTEGER(C_INT) FUNCTION selfamd_vector_dg_divergence_2d(dgM, bM, qW, f, bF, dF, N, nVar, nEl, stream) &
         BIND(C, name="selfamd_vector_dg_divergence_2d")
       IMPORT :: C_INT, C_PTR

@@ -82,3 +82,59 @@ def test_fortran_driver_on_hopr_mesh(tmp_path):
     assert rel_err(tend, c.om.tendency()).max() <= TOL
     c.om.rk3_steps(steps, dt)
     assert rel_err(out, c.om.solution).max() <= TOL
+
+
+def _read_tecplot(path):
+    """Header, zone lines and the data rows (float array per zone) of a WriteTecplot file."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    header, zones, rows = lines[0], [], []
+    for line in lines[1:]:
+        if line.lstrip().startswith("ZONE"):
+            zones.append(line)
+            rows.append([])
+        else:
+            rows[-1].append([float(v) for v in line.split()])
+    return header, zones, [np.array(r) for r in rows]
+
+
+def test_write_tecplot_fortran_and_python(tmp_path):
+    """WriteTecplot (src/SELF_Model2D.f90:1101-1207) through the Fortran layer and through
+    Python on the same state: the files are identical byte for byte, and the values are the
+    element positions and the solution interpolated to the uniform (M+1)^2 plot grid."""
+    import self_amd
+    N, M, n, steps = 5, 8, 3, 4
+    c = Case("sw", N, n, n, periodic=True, bcid=0, seed=8, metrics="affine")
+    prefix = str(tmp_path / "tec")
+    c.sol.astype(np.float64).tofile(prefix + "_sol.bin")
+    np.ascontiguousarray(c.fields["topo"]).tofile(prefix + "_field1.bin")
+    np.zeros_like(c.fields["topo"]).tofile(prefix + "_field2.bin")
+    par = tmp_path / "run.nml"
+    par.write_text(f"&run model=2, N={N}, Mtarget={M}, nx={n}, ny={n}, periodic=.true., bcid=0, nSteps={steps}, "
+                   f"dt={float(c.dt())!r}, nodal=.false., prefix='{prefix}' /\n")
+    r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(prefix + "_out.bin").reshape(c.sol.shape)
+    # the same state written from Python, on the geometry SEMQuad generates for the same mesh
+    mesh = self_amd.Mesh2D.structured(n, n, periodic=(True, True))
+    interp = self_amd.Lagrange(N, self_amd.GAUSS, M, self_amd.UNIFORM)
+    geom = self_amd.SEMQuad(interp, mesh)
+    m = self_amd.ShallowWater(mesh, geom)
+    m.SetSolution(out)
+    py = m.WriteTecplot(str(tmp_path / "py.tec"))
+    with open(prefix + ".tec") as a, open(py) as b:
+        assert a.read() == b.read()
+    header, zones, rows = _read_tecplot(py)
+    assert header == ' VARIABLES = "X", "Y", "Hu", "Hv", "H", "d/dx(Hu)", "d/dx(Hv)", "d/dx(H)", ' \
+                     '"d/dy(Hu)", "d/dy(Hv)", "d/dy(H)"'
+    assert zones == [f' ZONE T="el{e + 1:08d}", I={M + 1}, J={M + 1}' for e in range(n * n)]
+    iM = interp.iMatrix  # [M+1, N+1]: value at target point a = sum_i iM[a, i] f(i)
+    x = geom.x.to_host()
+    for e in range(n * n):
+        d = rows[e]
+        assert d.shape == ((M + 1) ** 2, 11)
+        xt = np.einsum("bj,ai,jic->bac", iM, iM, x[e]).reshape(-1, 2)
+        st = np.einsum("bj,ai,vji->bav", iM, iM, out[e]).reshape(-1, 3)
+        assert np.allclose(d[:, :2], xt, rtol=1e-7, atol=1e-7)
+        assert np.allclose(d[:, 2:5], st, rtol=1e-7, atol=1e-7 * np.abs(out).max())
+        assert not d[:, 5:].any()
