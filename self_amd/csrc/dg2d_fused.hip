@@ -312,7 +312,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // row-wise writes stay conflict-free.  Matrices hold entry (c, r) at c + MRS r, with two pad
   // doubles per 8-double row for the same reason (they are written once per block).
   constexpr bool SWZ = NP == 8;
-  constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NN;
+  // At N = 5 two pad doubles per element array (38 instead of 36) shift the elements of a
+  // tile against each other in the banks: 70.0 against 71.1 us per SW launch at 256^2 (row
+  // strides of 7 were slower, 72.3-72.7 us).
+  constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NP == 6 ? NN + 2 : NN;
   auto slot = [](int i_, int j_) { return SWZ ? NP * j_ + (i_ ^ (j_ & 6)) : i_ + NP * j_; };
   __shared__ double sQ[NT][EPB][NNS];
   __shared__ double sF1[NV][EPB][NNS];
