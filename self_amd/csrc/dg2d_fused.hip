@@ -379,17 +379,24 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // ---- loads: face lanes (neighbour traces, lagged velocities, normals) ----
   // SPLIT: wavefront w covers the face nodes of element w (NF <= 32 slots), lanes l and l + 32
   // share face node l; otherwise face node = thread index
+  // FPAD (N = 5, tiles of 128 or 256): each element's 24 face nodes take 32 lanes, south and
+  // north faces in lanes 0-11 and east and west in lanes 16-27 of the group, so that a 16-lane
+  // LDS group reads either columns or rows of the element, never both (no bank conflicts)
+  constexpr bool FPAD = !SPLIT && NF == 24 && 32 * EPB <= THREADS;
   const int fhalf = SPLIT ? (tid >> 5) & 1 : 0;
-  const int fslot = SPLIT ? (tid & 31) : 0;
-  const int fidx = SPLIT ? (tid >> 6) * NF + fslot : tid;
+  const int fslot = SPLIT ? (tid & 31) : FPAD ? (tid & 15) : 0;
+  const int fidx = SPLIT ? (tid >> 6) * NF + fslot : FPAD ? (tid >> 5) * NF + (fslot % 12) : tid;
   const int fel = fidx / NF, fslotN = fidx % NF;
   // SPLIT: slots take the sides in the order south, north, east, west, so that the lanes of a
   // 16-lane LDS group read opposite faces, whose face lines are the same nodes (no bank
   // conflicts between them)
-  const int side = SPLIT ? ((fslotN / NP) >> 1 | ((fslotN / NP) & 1) << 1) : fslotN / NP;
+  const int side = SPLIT  ? ((fslotN / NP) >> 1 | ((fslotN / NP) & 1) << 1)
+                   : FPAD ? ((tid >> 4) & 1) + (fslot >= NP ? 2 : 0)
+                          : fslotN / NP;
   const int fi = fslotN % NP, fnode = side * NP + fi;
   const int pf = blk * EPB + fel;
-  const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && fel < EPB && pf < a.nEl;
+  const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && (!FPAD || fslot < 2 * NP) && fel < EPB &&
+                          pf < a.nEl;
   int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
   if constexpr (SPLIT && NF == 32 && NN == 64) fe = __builtin_amdgcn_readfirstlane(fe);  // the wavefront's element
   // SPLIT: LDS slots of the half face line this lane interpolates (phases 1 and 3).  South and
@@ -645,6 +652,26 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
 #pragma unroll
         for (int k = 0; k < NT; k++) tr[(size_t)k * NP] = qn[k];
       }
+    }
+    return;
+  }
+  if constexpr (FPAD) {
+    // the face lanes of phase 1, in their padded order
+    if (faceActive) {
+      const double *bm = (side == 1 || side == 2) ? bM1 : bM0;
+      const bool alongJ = (side == 0 || side == 2);
+      double qn[NT];
+#pragma unroll
+      for (int k = 0; k < NT; k++) qn[k] = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < NP; ii++) {
+        const int nd = alongJ ? slot(fi, ii) : slot(ii, fi);
+#pragma unroll
+        for (int k = 0; k < NT; k++) qn[k] += bm[ii] * sQ[k][fel][nd];
+      }
+      double *tr = a.traceOut + (size_t)fi + NP * (size_t)NT * (side + 4 * (size_t)fe);
+#pragma unroll
+      for (int k = 0; k < NT; k++) tr[(size_t)k * NP] = qn[k];
     }
     return;
   }
