@@ -845,9 +845,26 @@ inline bool tunable(int kind, int N) {
          ((kind == SELFAMD_MODEL_LSW || kind == SELFAMD_MODEL_EULER) && N == 7);
 }
 inline int tile_of(int kind, int N) { return tunable(kind, N) ? g_tile[kind][N] : -1; }
-// element order alternates between stages (the default tiles alternate too)
-inline bool tile_alternates(int kind, int N) {
+// Launches of at most kSmallLaunch elements (a few per CU) take small tiles in plain element
+// order, whatever the table (tuned on 256^2 meshes) says.  Measured per launch: SW at N = 7,
+// 64 wide, 4.75 against 5.54 us at 10^2 and 10.8 against 14.7 us at 64^2 (the tuned 256-wide
+// alternating tile wins from 96^2 on); SW at N = 5, 128 wide in XCD order, 8.6 against 12.1 us
+// at 64^2 (equal at 128^2); LSW and Euler at N = 7, 64 wide, 4.4 against 5.1 us (10^2) and
+// 7.2 against 9.6 us (32^2).  The face sums split as the table's tile does (N = 5 keeps a
+// table entry of 64 wide, which splits), so a run's results do not depend on launch sizes.
+constexpr int kSmallLaunch = 8192;
+inline int tile_for(int kind, int N, int nEl) {
   const int v = tile_of(kind, N);
+  if (tunable(kind, N) && nEl <= kSmallLaunch) {
+    if (N == 7) return 0;
+    const bool split64 = v >= 0 && ((v < 12 && (v >> 2) == 0) || v == 12 || v == 13);
+    if (N == 5 && !split64) return 5;
+  }
+  return v;
+}
+// element order alternates between stages (the default tiles alternate too)
+inline bool tile_alternates(int kind, int N, int nEl) {
+  const int v = tile_for(kind, N, nEl);
   return v < 0 || v >= 12 || (v & 2);
 }
 int g_tileEpoch = 0;
@@ -876,7 +893,7 @@ hipError_t launch_tile(int v, const FusedArgs &a, hipStream_t st) {
 template <class Phys>
 struct Launcher {
   static hipError_t stage(int N, int mode, bool nodal, const FusedArgs &a, hipStream_t st) {
-    const int v = (mode == 0 && !nodal) ? tile_of(Phys::kKind, N) : -1;
+    const int v = (mode == 0 && !nodal) ? tile_for(Phys::kKind, N, a.nEl) : -1;
     if (v >= 0) {
       if constexpr (std::is_same<Phys, PhysSW>::value) {
         if (N == 3) return launch_tile<Phys, 3>(v, a, st);
@@ -999,7 +1016,7 @@ struct selfamd_model2d {
     a.rkG = kRk3G[stage];
     a.writeG = stage != 2;
     // element order alternates with the trace buffer parity, i.e. from stage to stage
-    a.reverse = tile_alternates(kind, N) ? traceIn : 0;
+    a.reverse = tile_alternates(kind, N, a.nEl) ? traceIn : 0;
     a.perElemDt = 0;
     a.p = p;
     switch (kind) {
@@ -1627,7 +1644,7 @@ struct selfamd_batch {
     a.geom = g.geom; a.nbr = g.nbr; a.mats = g.mats; a.nEl = g.nEl;
     a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
     a.writeG = stage != 2;
-    a.reverse = tile_alternates(kind, g.N) ? tin : 0;
+    a.reverse = tile_alternates(kind, g.N, g.nEl) ? tin : 0;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);
