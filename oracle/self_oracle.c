@@ -9,16 +9,23 @@
  * each pass may run under OpenMP (or_set_threads); passes are element-independent, so the
  * threaded result is bit-identical to the serial one.
  *
- * Parity pinning (what checks each group, see tests/test_oracle.py):
+ * Parity pinning (what checks each group; tests/test_oracle*.py):
  *   quadrature/Lagrange  -> numpy leggauss closed forms, D*1=0, D*x=1, exactness, dg identity
  *   interp/divergence    -> polynomial exactness, weak==strong for interpolated normal flux
  *   connectivity         -> HOPr mesh fixtures derived from the etc/mesh HOPr files (h5dump)
  *   side exchange        -> brute-force permutation identity on structured meshes
- *   Riemann/flux/source  -> hand-evaluated closed forms and symmetry F(L,R,n) = -F(R,L,-n)
+ *   Riemann solvers      -> consistency F*(q,q,n) = F(q).n nScale and conservation
+ *                           F*(L,R,n) = -F*(R,L,-n) against the physical fluxes; the SW and
+ *                           Euler dissipation coefficients against eigenvalues of
+ *                           finite-difference flux Jacobians; LSW = exact Godunov flux
+ *   wall BC              -> reflection u - 2(u.n)n for arbitrary unit normals (Circle mesh),
+ *                           mass conservation on walled meshes
+ *   volume flux, source  -> the tendency of a smooth state converges to -div F + S of the
+ *                           PDE; closed-form tendency of a uniform state (Euler, quirk Q3)
  *   tendency             -> mass conservation, lake at rest, free-stream preservation
- *   RK3                  -> third-order convergence on y' = lambda*y
- * Parity unpinned (read-only restatement, no executable check possible): the SW one-stage
- * velocity-trace lag (quirk Q1) and the Euler energy source term (quirk Q3).
+ *   SW velocity lag (Q1) -> first-order temporal convergence of the nonlinear RK3 step
+ *   RK3                  -> third-order convergence of the linear semi-discrete system
+ * No function on the tendency path is left unpinned.
  */
 #include "self_oracle.h"
 #include <math.h>
