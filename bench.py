@@ -163,25 +163,64 @@ def setup_single(self_amd, workloads, name, seed):
                 nVar=nVar, NT=NT, p=p, steps=steps)
 
 
-def cpu_single(oracle, name, w, steps):
-    threads = min(16, os.cpu_count() or 1)
-    og = oracle.Geometry(w["N"], w["mesh"].nodeCoords, 1)
-    kind = {"sw256": oracle.SW, "euler512": oracle.EULER, "lsw10": oracle.LSW}[name]
-    om = oracle.Model(kind, w["N"], w["mesh"].sideInfo, og)
-    for k, v in w["fields"].items():
-        {"topography": om.set_topography, "potential": om.set_gravity, "bathymetry": om.set_bathymetry}[k](v)
-    om.set_solution(w["sol"])
-    if name == "euler512":
-        om.set_prescribed()
-    oracle.set_threads(threads)
-    c0 = time.perf_counter()
-    om.rk3_steps(steps, w["dt"])
-    csec = time.perf_counter() - c0
-    oracle.set_threads(1)
-    dof = w["mesh"].nElem * (w["N"] + 1) ** 2 * w["nVar"]
-    return {"value": dof * steps / csec, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} RK3 step(s) of the same {w['nx']}x{w['nx']} N={w['N']} workload, "
-                      f"OpenMP over elements, {csec:.2f} s"}
+def host_info():
+    """The host's CPU model, core counts and the threads the CPU baseline may use: all cores
+    this process may run on, capped by OMP_NUM_THREADS where the machine sets it (the GPU
+    hosts give one GPU's job 16 cores that way)."""
+    model, flags = "unknown", ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = line.split(":", 1)[1]
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(affinity, omp) if omp > 0 else affinity
+    return {"cpu_model": model, "nproc": nproc, "affinity_cores": affinity, "omp_num_threads": omp or None,
+            "threads": threads, "avx512": " avx512f " in f" {flags} "}
+
+
+def cpu_single(oracle, name, w, steps_all, steps_one):
+    """The oracle's C restatement of the reference CPU path, built as an optimising compiler
+    would (-O3, FMA contraction, AVX-512 where the host has it), timed on this host: once on
+    one core (the reference has no threading) and once with OpenMP over elements on all the
+    cores this job may use (the single-node stand-in for the reference's MPI decomposition)."""
+    info = host_info()
+    build = "fast_v4" if info["avx512"] else "fast_v3"
+    oracle.use_build(build)
+    try:
+        og = oracle.Geometry(w["N"], w["mesh"].nodeCoords, 1)
+        kind = {"sw256": oracle.SW, "euler512": oracle.EULER, "lsw10": oracle.LSW}[name]
+        om = oracle.Model(kind, w["N"], w["mesh"].sideInfo, og)
+        for k, v in w["fields"].items():
+            {"topography": om.set_topography, "potential": om.set_gravity, "bathymetry": om.set_bathymetry}[k](v)
+        om.set_solution(w["sol"])
+        if name == "euler512":
+            om.set_prescribed()
+        dof = w["mesh"].nElem * (w["N"] + 1) ** 2 * w["nVar"]
+        rates = {}
+        for threads, steps in ((1, steps_one), (info["threads"], steps_all)):
+            oracle.set_threads(threads)
+            c0 = time.perf_counter()
+            om.rk3_steps(steps, w["dt"])
+            rates[threads] = (dof * steps / (time.perf_counter() - c0), steps, time.perf_counter() - c0)
+        oracle.set_threads(1)
+        del om, og
+    finally:
+        oracle.use_build("parity")
+    allr, one = rates[info["threads"]], rates[1]
+    return {"value": allr[0], "unit": "DOF-updates/s", "cores": info["threads"], "kind": "port",
+            "value_1core": one[0], "build": f"oracle/{oracle.BUILDS[build].rsplit('/', 1)[-1]}",
+            "cpu_model": info["cpu_model"], "nproc": info["nproc"], "affinity_cores": info["affinity_cores"],
+            "omp_num_threads": info["omp_num_threads"],
+            "sample": f"RK3 steps of the same {w['nx']}x{w['nx']} N={w['N']} workload: {allr[1]} on "
+                      f"{info['threads']} threads (OpenMP over elements) in {allr[2]:.2f} s, {one[1]} on 1 core "
+                      f"in {one[2]:.2f} s"}
 
 
 def run_single(a, root):
@@ -211,7 +250,9 @@ def run_single(a, root):
     cpu = None
     if not a.no_cpu_baseline:
         import oracle
-        cpu = cpu_single(oracle, a.workload, w, a.cpu_steps or {"sw256": 100, "euler512": 5, "lsw10": 200}[a.workload])
+        steps_all = a.cpu_steps or {"sw256": 100, "euler512": 5, "lsw10": 200}[a.workload]
+        steps_one = max(1, steps_all // {"sw256": 10, "euler512": 5, "lsw10": 2}[a.workload])
+        cpu = cpu_single(oracle, a.workload, w, steps_all, steps_one)
     phys, kind = {"sw256": ("PhysSW", 2), "euler512": ("PhysEuler", 3), "lsw10": ("PhysLSW", 1)}[a.workload]
     kernel = tile_name(self_amd, N, phys, kind)
     traffic = traffic_from_profiles(root, {"sw256": "sw2d_stage_256x256_N7_traffic.json",

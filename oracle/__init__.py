@@ -21,6 +21,13 @@ GAUSS, GAUSS_LOBATTO, UNIFORM = 1, 2, 5
 LSW, SW, EULER = 1, 2, 3
 BC_PRESCRIBED, BC_RADIATION, BC_NONORMALFLOW = 100, 101, 102
 
+# builds of the same source (oracle/Makefile): "parity" is the oracle proper; the "fast"
+# builds allow FMA contraction, as an optimising compiler would, and serve only as the timed
+# CPU baseline of bench.py
+BUILDS = {"parity": LIB_PATH, "fast_v3": os.path.join(_HERE, "liboracle_fast_v3.so"),
+          "fast_v4": os.path.join(_HERE, "liboracle_fast_v4.so")}
+_libs = {}
+_build = "parity"
 _lib = None
 _vp = ctypes.c_void_p
 
@@ -29,12 +36,25 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_build(name):
+    """Selects the library that objects created from now on use (existing objects keep
+    theirs)."""
+    global _build, _lib
+    if name not in BUILDS:
+        raise ValueError(f"unknown oracle build {name!r}")
+    _build = name
+    _lib = None
+
+
 def lib():
     global _lib
+    if _lib is None and _build in _libs:
+        _lib = _libs[_build]
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = BUILDS[_build]
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.or_geometry_from_mesh.restype = _vp
         L.or_model_create.restype = _vp
         L.or_model_field.restype = ctypes.POINTER(ctypes.c_double)
@@ -59,6 +79,7 @@ def lib():
         for f in ("or_model_set_solution", "or_sw_set_topography", "or_lsw_set_bathymetry", "or_lsw_set_coriolis",
                   "or_euler_set_gravity", "or_euler_set_drag"):
             getattr(L, f).argtypes = [_vp, _vp]
+        _libs[_build] = L
         _lib = L
     return _lib
 
@@ -225,21 +246,22 @@ class Geometry:
     def __init__(self, N, nodeCoords, nGeo, ctrl=GAUSS):
         nc = np.ascontiguousarray(nodeCoords, np.float64)
         self.N, self.nElem = N, nc.shape[0]
-        self._p = ctypes.c_void_p(lib().or_geometry_from_mesh(N, ctrl, nGeo, self.nElem, P(nc)))
+        self._L = lib()
+        self._p = ctypes.c_void_p(self._L.or_geometry_from_mesh(N, ctrl, nGeo, self.nElem, P(nc)))
         p, n = N + 1, self.nElem
         shapes = dict(x=(n, p, p, 2), xb=(n, 4, p, 2), dxds=(n, p, p, 2, 2), dxdsb=(n, 4, p, 2, 2),
                       dsdx=(n, p, p, 2, 2), dsdxb=(n, 4, p, 2, 2), nHatb=(n, 4, p, 2), nScaleb=(n, 4, p),
                       J=(n, p, p), Jb=(n, 4, p))
         for k, shp in shapes.items():
-            ptr = lib().or_geometry_field(self._p, k.encode())
+            ptr = self._L.or_geometry_field(self._p, k.encode())
             setattr(self, k, np.ctypeslib.as_array(ptr, shape=shp))
 
     def covariant_arc_min(self):
-        return lib().or_covariant_arc_min(self._p)
+        return self._L.or_covariant_arc_min(self._p)
 
     def __del__(self):
-        if getattr(self, "_p", None) is not None and _lib is not None:
-            _lib.or_geometry_free(self._p)
+        if getattr(self, "_p", None) is not None:
+            self._L.or_geometry_free(self._p)
             self._p = None
 
 
@@ -251,15 +273,16 @@ class Model:
         self.sideInfo = np.ascontiguousarray(sideInfo, np.int32)
         self.nElem = self.sideInfo.shape[0]
         self.nVar = 4 if kind == EULER else 3
-        self._p = ctypes.c_void_p(lib().or_model_create(kind, N, self.nElem, P(self.sideInfo), geom._p))
+        self._L = geom._L
+        self._p = ctypes.c_void_p(self._L.or_model_create(kind, N, self.nElem, P(self.sideInfo), geom._p))
 
     def __del__(self):
-        if getattr(self, "_p", None) is not None and _lib is not None:
-            _lib.or_model_free(self._p)
+        if getattr(self, "_p", None) is not None:
+            self._L.or_model_free(self._p)
             self._p = None
 
     def field(self, name, shape):
-        return np.ctypeslib.as_array(lib().or_model_field(self._p, name.encode()), shape=shape)
+        return np.ctypeslib.as_array(self._L.or_model_field(self._p, name.encode()), shape=shape)
 
     def _scal(self, x, nvar=1):
         p, n = self.N + 1, self.nElem
@@ -277,51 +300,51 @@ class Model:
         return (self.nElem, 4, self.nVar, self.N + 1)
 
     def set_scalar(self, name, v):
-        lib().or_model_set_scalar(self._p, name.encode(), float(v))
+        self._L.or_model_set_scalar(self._p, name.encode(), float(v))
 
     def get_scalar(self, name):
-        return lib().or_model_get_scalar(self._p, name.encode())
+        return self._L.or_model_get_scalar(self._p, name.encode())
 
     def set_solution(self, interior):
         a = np.ascontiguousarray(np.asarray(interior, np.float64).reshape(self.interior_shape))
-        lib().or_model_set_solution(self._p, P(a))
+        self._L.or_model_set_solution(self._p, P(a))
 
     def set_topography(self, b):
         a = self._scal(b)  # named: must outlive the call
-        lib().or_sw_set_topography(self._p, P(a))
+        self._L.or_sw_set_topography(self._p, P(a))
 
     def set_bathymetry(self, H):
         a = self._scal(H)  # named: must outlive the call
-        lib().or_lsw_set_bathymetry(self._p, P(a))
+        self._L.or_lsw_set_bathymetry(self._p, P(a))
 
     def set_coriolis(self, f):
         a = self._scal(f)  # named: must outlive the call
-        lib().or_lsw_set_coriolis(self._p, P(a))
+        self._L.or_lsw_set_coriolis(self._p, P(a))
 
     def set_gravity(self, phi):
         a = self._scal(phi)  # named: must outlive the call
-        lib().or_euler_set_gravity(self._p, P(a))
+        self._L.or_euler_set_gravity(self._p, P(a))
 
     def set_drag(self, cd):
         a = self._scal(cd)  # named: must outlive the call
-        lib().or_euler_set_drag(self._p, P(a))
+        self._L.or_euler_set_drag(self._p, P(a))
 
     def set_prescribed(self):
-        lib().or_euler_set_prescribed(self._p)
+        self._L.or_euler_set_prescribed(self._p)
 
     @property
     def solution(self):
         return self.field("sol", self.interior_shape).copy()
 
     def tendency(self):
-        lib().or_calculate_tendency(self._p)
+        self._L.or_calculate_tendency(self._p)
         return self.field("dSdt", self.interior_shape).copy()
 
     def rk3_steps(self, n, dt):
-        lib().or_rk3_steps(self._p, int(n), float(dt))
+        self._L.or_rk3_steps(self._p, int(n), float(dt))
 
     def entropy(self):
-        return lib().or_entropy(self._p)
+        return self._L.or_entropy(self._p)
 
     def mass(self, var):
-        return lib().or_mass(self._p, int(var))
+        return self._L.or_mass(self._p, int(var))

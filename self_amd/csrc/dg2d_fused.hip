@@ -36,6 +36,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kNbrOrphan = -0x40000000;
+// forward_step calls of at most this many steps with a step size other than the captured
+// graphs' are launched stage by stage instead of recapturing
+constexpr int kDirectSteps = 2;
 
 // Williamson low-storage RK3 (src/SELF_Model.f90:25-27)
 constexpr double kRk3A[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
@@ -1068,6 +1071,30 @@ struct selfamd_model2d {
     (void)hipGraphDestroy(g);
     return ie;
   }
+  // `steps` RK3 steps launched stage by stage on the stream, without a graph
+  hipError_t launch_steps(int steps, double dt) {
+    for (int n = 0; n < steps; n++) {
+      if (nBands > 0) {
+        for (int r = 0; r < nBands + 2; r++)
+          for (int st = 0; st < 3; st++) {
+            const int b = r - st;
+            if (b >= 0 && b < nBands) {
+              hipError_t e = launch_stage(0, st, dt, cur ^ (st & 1), nullptr, b);
+              if (e != hipSuccess) return e;
+            }
+          }
+        cur ^= 1;
+      } else {
+        for (int st = 0; st < 3; st++) {
+          hipError_t e = launch_stage(0, st, dt, cur, nullptr);
+          if (e != hipSuccess) return e;
+          cur ^= 1;
+        }
+      }
+      t += dt;
+    }
+    return hipSuccess;
+  }
 };
 
 using Model = selfamd_model2d;
@@ -1378,6 +1405,12 @@ int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {
 int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
   SELFAMD_REQUIRE(m && nSteps >= 0 && dt > 0.0, "model2d_forward_step: bad arguments (nSteps=%d dt=%g)", nSteps, dt);
   if (nSteps == 0) return kOk;
+  if (m->graphDt > 0.0 && m->graphDt != dt && m->graphEpoch == g_tileEpoch && nSteps <= kDirectSteps) {
+    // a few steps of another size (e.g. ForwardStep's shorter last step up to tn): launched
+    // directly, so the graphs of the running step size survive
+    SELFAMD_HIP(m->launch_steps(nSteps, dt));
+    return kOk;
+  }
   if (m->graphDt != dt || m->graphEpoch != g_tileEpoch) {
     m->drop_graphs();
     for (int start = 0; start < 2; start++) SELFAMD_HIP(m->capture(start, 1, dt, &m->graph[start]));
@@ -1681,6 +1714,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
     const Model *m = models[r];
     SELFAMD_REQUIRE(m && m->kind == m0->kind, "batch_create: run %d: all runs must be the same model kind", r);
     SELFAMD_REQUIRE(!m->nodal, "batch_create: run %d uses nodal metrics; batched runs take affine meshes", r);
+    SELFAMD_REQUIRE(!m->closure, "batch_create: run %d has a closure attached; batched launches do not apply it", r);
     SELFAMD_REQUIRE(dt[r] > 0.0, "batch_create: run %d: dt must be positive", r);
     SELFAMD_REQUIRE(memcmp(&m->p, &m0->p, sizeof(Params)) == 0,
                     "batch_create: run %d: physical parameters differ from run 0", r);
@@ -1693,10 +1727,10 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
   b->p = m0->p;
   b->models.assign(models, models + nModels);
   b->dt.assign(dt, dt + nModels);
-  auto fail = [&](const char *what) {
+  auto fail = [&](const char *what, int code = kHipError) {
     set_error("batch_create: %s", what);
     selfamd_batch_destroy(b);
-    return (int)kHipError;
+    return code;
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed");
   for (int N = 1; N <= 7; N++) {
@@ -1706,7 +1740,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
     for (int r = 0; r < nModels; r++)
       if (models[r]->N == N) { g.runs.push_back(r); g.first.push_back(total); total += models[r]->nEl; }
     if (g.runs.empty()) continue;
-    SELFAMD_REQUIRE(total < (size_t)1 << 27, "batch_create: degree-%d group has too many elements", N);
+    if (total >= (size_t)1 << 27) return fail("a degree group has too many elements (at most 2^27)", kBadArgument);
     g.nEl = (int)total;
     const size_t P = N + 1, PP = P * P, nV = m0->nVar, nT = m0->nT;
     const size_t nS = total * PP * nV, nTr = total * P * nT * 4;

@@ -9,18 +9,17 @@ KIND = {"lsw": oracle.LSW, "sw": oracle.SW, "euler": oracle.EULER}
 
 
 def rel_err(a, b, axis_var=1):
-    """Per-variable relative max-norm max|a-b|/max|b| (variables along axis 1).  A variable
-    whose reference magnitude is below 1e-10 of the largest one is measured against the
-    largest one instead (e.g. a tendency that is zero up to round-off)."""
+    """Per-variable relative max-norm max|a_v - b_v| / max|b_v| (variables along axis_var).
+    Every variable is judged against its own magnitude; a variable whose reference is
+    identically zero must match exactly (its entry is 0 if a_v is zero too, else inf)."""
     a = np.asarray(a)
     b = np.asarray(b)
-    nv = b.shape[axis_var]
-    ref = [np.abs(np.take(b, v, axis=axis_var)).max() for v in range(nv)]
-    big = max(ref) if max(ref) > 0 else 1.0
     out = []
-    for v in range(nv):
-        d = np.abs(np.take(a, v, axis=axis_var) - np.take(b, v, axis=axis_var)).max()
-        out.append(d / (ref[v] if ref[v] > 1e-10 * big else big))
+    for v in range(b.shape[axis_var]):
+        bv, av = np.take(b, v, axis=axis_var), np.take(a, v, axis=axis_var)
+        ref = np.abs(bv).max()
+        d = np.abs(av - bv).max()
+        out.append(d / ref if ref > 0 else (0.0 if d == 0 else np.inf))
     return np.array(out)
 
 

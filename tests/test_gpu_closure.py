@@ -182,3 +182,17 @@ def test_closure_training_gain_on_held_out_runs():
     assert mean[alpha] <= mean[0.0]
     assert np.isfinite(c["err"]).all()
     assert abs(c["mass_drift"]) < 1e-13
+
+
+def test_batch_rejects_a_model_with_a_closure():
+    """Batched launches do not apply a closure, so a model carrying one cannot join a batch."""
+    import self_amd
+    m, *_ = _coarse_model(8, 3, seed=3)
+    net, _ = _mlp(5)
+    m.SetClosure(net, *_norm(5))
+    with pytest.raises(self_amd.SelfAmdError):
+        self_amd.Batch([m], [1e-3])
+    m.SetClosure(None)
+    b = self_amd.Batch([m], [1e-3])
+    b.ForwardStep(1)
+    b.scatter()

@@ -125,6 +125,23 @@ def test_multi_step_graph_matches_single_steps(kind):
     assert rel_err(a.solution, c.om.solution).max() <= TOL
 
 
+@pytest.mark.parametrize("bands", [0, 3])
+def test_short_step_of_another_size_keeps_graphs(bands):
+    """ForwardStep's shorter last step (a dt other than the captured graphs') runs stage by
+    stage without recapturing; the trajectory across full, short and full steps matches the
+    oracle, with and without banded stepping."""
+    c = Case("sw", 5, 12, 12, periodic=True, seed=6, metrics="affine")
+    m, _ = c.gpu_model()
+    if bands:
+        m.SetBands(bands)
+    dt = c.dt()
+    for n, h in ((9, dt), (1, 0.37 * dt), (2, 0.5 * dt), (9, dt)):
+        m.ForwardStep(n, h)
+        c.om.rk3_steps(n, h)
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+    assert abs(m.t - (18 * dt + 0.37 * dt + dt)) <= 1e-15
+
+
 @pytest.mark.parametrize("kind,n,periodic,bands", [("sw", 24, True, 4), ("sw", 24, True, 9), ("euler", 16, False, 5),
                                                    ("lsw", 20, False, 3)])
 def test_banded_steps_bit_identical(kind, n, periodic, bands):
@@ -211,6 +228,26 @@ def test_full_size_sw_256():
     s = m.solution
     assert np.isfinite(s).all()
     assert abs(mass(s) - m0) <= 1e-13 * m0
+
+
+def test_full_size_sw_256_reference_metrics():
+    """The headline configuration on the reference's own metric representation: 256^2
+    elements, N = 7, periodic, geometry generated on the device by SEMQuad (per-node metric
+    terms, bit-identical to the oracle's), the engine in per-node-metric mode, against the
+    unmodified oracle for two RK3 steps."""
+    import os
+    c = Case("sw", 7, 256, 256, periodic=True, seed=0, metrics="nodal")
+    m, geom = c.gpu_model(geometry="device")
+    for k in ("dsdx", "J", "nHatb", "nScaleb"):
+        assert np.array_equal(getattr(geom, k).to_host(), getattr(c.geom, k)), k
+    dt = c.dt()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        c.om.rk3_steps(2, dt)
+    finally:
+        oracle.set_threads(1)
+    m.ForwardStep(2, dt)
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
 def test_lsw_10x10_full_run():

@@ -10,44 +10,21 @@ is merely compiled with FMA contraction (-ffp-contract=fast, which an optimising
 FMA-capable CPU may apply) instead of without it.  Both are round-off effects of the same
 size: beyond this level the reference result is not reproducible across legitimate builds.
 """
-import os
-import subprocess
-import sys
-
 import numpy as np
-import pytest
 
 import oracle
 from tests.cases import Case, rel_err
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 N, NX = 7, 64
 
-CHILD = r"""
-import sys
-import numpy as np
-sys.path.insert(0, {root!r})
-import oracle
-oracle.LIB_PATH = {lib!r}
-from tests.cases import Case
-c = Case("sw", {N}, {NX}, {NX}, periodic=True, seed=0, metrics="nodal", generic=False)
-np.save({out!r}, c.om.tendency())
-"""
 
-
-def test_affine_metrics_within_build_reproducibility(tmp_path):
-    lib = str(tmp_path / "liboracle_fma.so")
-    src = os.path.join(ROOT, "oracle", "self_oracle.c")
-    cc = subprocess.run(["gcc", "-O3", "-march=x86-64-v3", "-fPIC", "-fopenmp", "-fno-fast-math",
-                         "-ffp-contract=fast", "-std=c11", "-shared", "-o", lib, src, "-lm"],
-                        capture_output=True, text=True)
-    if cc.returncode != 0:
-        pytest.skip(f"gcc could not build the contracted oracle: {cc.stderr[-300:]}")
-    out = str(tmp_path / "fma.npy")
-    code = CHILD.format(root=ROOT, lib=lib, N=N, NX=NX, out=out)
-    subprocess.run([sys.executable, "-c", code], check=True, timeout=600)
-    d_fma_run = np.load(out)
-
+def test_affine_metrics_within_build_reproducibility():
+    # the FMA-contracted build of the same source (oracle/Makefile, liboracle_fast_v3.so)
+    oracle.use_build("fast_v3")
+    try:
+        d_fma_run = Case("sw", N, NX, NX, periodic=True, seed=0, metrics="nodal", generic=False).om.tendency()
+    finally:
+        oracle.use_build("parity")
     nodal = Case("sw", N, NX, NX, periodic=True, seed=0, metrics="nodal", generic=False).om.tendency()
     affine = Case("sw", N, NX, NX, periodic=True, seed=0, metrics="affine", generic=False).om.tendency()
     d_fma = rel_err(d_fma_run, nodal).max()
