@@ -283,6 +283,12 @@ int selfamd_model2d_set_solution(selfamd_model2d *model, const double *interior_
 int selfamd_model2d_get_solution(selfamd_model2d *model, double *interior_dev);
 /* CalculateTendency (src/SELF_Model2D.f90:816-859): dSdt(0:N,0:N,nVar,nEl) */
 int selfamd_model2d_tendency(selfamd_model2d *model, double *dSdt_dev);
+/* The flux and fluxDivergence registers that CalculateTendency_Model2D
+ * (src/SELF_Model2D.f90:816-859) leaves behind, for the current state: flux(2,0:N,0:N,nVar,nEl)
+ * after ContravariantProjection and fluxDivergence(0:N,0:N,nVar,nEl) after JacobianWeight, as
+ * Write_Model2D (:861-1059) stores them.  Neither the state nor the SW lag state changes.
+ * Synchronises the model's stream. */
+int selfamd_model2d_flux_registers(selfamd_model2d *model, double *fluxDivergence_dev, double *flux_dev);
 /* CalculateEntropy of the current state into *entropy (host): SW
  * src/models/SELF_ShallowWater.f90:149-188, LSW src/models/SELF_LinearShallowWater.f90:171-216,
  * Euler src/models/SELF_CompressibleIdealGas2D.f90:782-823.  Euler takes the pressure of the

@@ -44,11 +44,22 @@ int selfamd_hopr_read_double(const char *path, const char *name, double *out, in
  * /state/interior/solution (0:N,0:N,nVar,nEl), /state/boundary/solution (0:N,nVar,4,nEl),
  * /mesh/interior/x (2,0:N,0:N,1,nEl), /mesh/boundary/x (2,0:N,1,4,nEl), plus the root
  * attribute N, which Read_Model2D reads (:1078) and the reference never writes.  The
- * fluxDivergence, flux and solutionGradient datasets are not written: the fused engine does
- * not form them, and ReadModel does not read them.  An existing file is replaced. */
+ * registers the reference also writes go in with selfamd_pickup_write_registers.  An existing
+ * file is replaced. */
 int selfamd_pickup_write(const char *path, int N, int nVar, int nEl, const double *xi, const double *weights,
                          const double *dgMatrix, const double *dMatrix, const double *bMatrix, const double *solution,
                          const double *solutionBoundary, const double *x, const double *xBoundary);
+
+/* The remaining datasets of Write_Model2D (src/SELF_Model2D.f90:981-990), added to a file
+ * that selfamd_pickup_write created: /state/interior/fluxDivergence (0:N,0:N,nVar,nEl),
+ * /state/interior/flux (2,0:N,0:N,nVar,nEl), /state/boundary/flux (2,0:N,nVar,4,nEl),
+ * /state/interior/solutionGradient (2,0:N,0:N,nVar,nEl) and
+ * /state/boundary/solutionGradient (2,0:N,nVar,4,nEl), dimensions reversed as above.  A null
+ * array is written as zeros (the reference's tendency path never fills the flux traces or the
+ * solution gradient). */
+int selfamd_pickup_write_registers(const char *path, int N, int nVar, int nEl, const double *fluxDivergence,
+                                   const double *flux, const double *fluxBoundary, const double *solutionGradient,
+                                   const double *solutionGradientBoundary);
 
 /* ReadModel (Read_Model2D, src/SELF_Model2D.f90:1061-1099): checks the attribute N against N
  * and the solution's dimensions, then reads /state/interior/solution (0:N,0:N,nVar,nEl). */

@@ -366,6 +366,22 @@ class Model2D:
                                                                interp.dMatrix, interp.bMatrix, sol, bnd.to_host(), x, xb)]
         call_io("selfamd_pickup_write", os.fsencode(os.path.abspath(path)), self.N, self.nVar, self.nElem,
                 *[a.ctypes.data for a in host])
+        # the flux and fluxDivergence registers of a tendency evaluation of the current state;
+        # the flux traces and the solution gradient, which the reference's tendency path never
+        # fills, are written as zeros
+        fdiv, fl = self.flux_registers()
+        call_io("selfamd_pickup_write_registers", os.fsencode(os.path.abspath(path)), self.N, self.nVar, self.nElem,
+                fdiv.ctypes.data, fl.ctypes.data, None, None, None)
+
+    def flux_registers(self):
+        """(fluxDivergence[e, v, j, i], flux[e, v, j, i, 2]): the registers CalculateTendency
+        (src/SELF_Model2D.f90:816-859) leaves for the current state, the contravariant flux and
+        its weak divergence divided by J.  The state and the SW lag state are unchanged."""
+        P = self.N + 1
+        fdiv = DeviceArray(self.shape)
+        fl = DeviceArray((self.nElem, self.nVar, P, P, 2))
+        call("selfamd_model2d_flux_registers", self._h, _ptr(fdiv), _ptr(fl))
+        return fdiv.to_host(), fl.to_host()
 
     def WriteTecplot(self, filename=None):
         """WriteTecplot (WriteTecplot_Model2D, src/SELF_Model2D.f90:1101-1207): the element

@@ -78,6 +78,7 @@ SIGNATURES = {
     "selfamd_model2d_set_solution": [c_void_p, c_void_p],
     "selfamd_model2d_get_solution": [c_void_p, c_void_p],
     "selfamd_model2d_tendency": [c_void_p, c_void_p],
+    "selfamd_model2d_flux_registers": [c_void_p, c_void_p, c_void_p],
     "selfamd_model2d_entropy": [c_void_p, c_void_p],
     "selfamd_model2d_integral": [c_void_p, c_int, c_void_p],
     "selfamd_model2d_forward_step": [c_void_p, c_int, c_double],
@@ -154,6 +155,7 @@ IO_SIGNATURES = {
     "selfamd_hopr_read_int": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
     "selfamd_hopr_read_double": [ctypes.c_char_p, ctypes.c_char_p, c_void_p, ctypes.c_int64],
     "selfamd_pickup_write": [ctypes.c_char_p, c_int, c_int, c_int] + [c_void_p] * 9,
+    "selfamd_pickup_write_registers": [ctypes.c_char_p, c_int, c_int, c_int] + [c_void_p] * 5,
     "selfamd_pickup_read": [ctypes.c_char_p, c_int, c_int, c_int, c_void_p],
 }
 _io = None

@@ -77,6 +77,10 @@ struct FusedArgs {
   const int32_t *elems;     // the launch's elements (a band), or null: elements 0 .. nEl-1
   const double *corr;       // learned tendency correction (interior layout), or null
   const double *corrShift;  // global mean of the mass-variable correction (one double)
+  // MODE 1 only, or null: the contravariant flux (2,0:N,0:N,nVar,nEl) and its weak divergence
+  // divided by J (0:N,0:N,nVar,nEl), the flux and fluxDivergence registers of
+  // CalculateTendency_Model2D (src/SELF_Model2D.f90:816-859)
+  double *fluxOut, *fluxDivOut;
   Params p;
 };
 
@@ -452,6 +456,11 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       // ContravariantProjection (src/SELF_MappedData.f90:2220-2269)
       sF1[v][el][nl] = dsdx[0] * fx[v] + dsdx[1] * fy[v];
       sF2[v][el][nl] = dsdx[2] * fx[v] + dsdx[3] * fy[v];
+      if (MODE == 1 && a.fluxOut) {
+        double *fo = a.fluxOut + 2 * (sBase + (size_t)v * NN + node);
+        fo[0] = sF1[v][el][nl];
+        fo[1] = sF2[v][el][nl];
+      }
     }
 #pragma unroll
     for (int k = 0; k < NT; k++) sQ[k][el][nl] = q[k];
@@ -606,6 +615,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) * iwi +
           (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) * iwj;
       d = d * invJ;
+      if (MODE == 1 && a.fluxDivOut) a.fluxDivOut[sBase + (size_t)v * NN + node] = d;
       double dsdt = src[v] - d;
       if (a.corr) {
         // closure: mass correction made mean-free over the mesh (variable 2 is H / eta / rho)
@@ -994,8 +1004,11 @@ struct selfamd_model2d {
   std::vector<int> bandStart;     // nBands + 1 offsets into bandElems
   std::vector<int32_t> hnbr;      // host copy of the neighbour codes
 
-  hipError_t launch_stage(int mode, int stage, double dt, int traceIn, double *dSdt, int band = -1) {
+  hipError_t launch_stage(int mode, int stage, double dt, int traceIn, double *dSdt, int band = -1,
+                          double *fluxOut = nullptr, double *fluxDivOut = nullptr) {
     FusedArgs a{};
+    a.fluxOut = fluxOut;
+    a.fluxDivOut = fluxDivOut;
     if (closure) {
       hipError_t e = closure_evaluate(*closure, S, geom, mats + 2 * (N + 1) * (N + 1) + 2 * (N + 1), N, nVar, 2, nEl,
                                       stream);
@@ -1399,6 +1412,20 @@ int selfamd_model2d_tendency(selfamd_model2d *m, double *dSdt) {
                                hipMemcpyDeviceToDevice, m->stream));
   }
   SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+int selfamd_model2d_flux_registers(selfamd_model2d *m, double *fluxDivergence, double *flux) {
+  SELFAMD_REQUIRE(m && fluxDivergence && flux, "model2d_flux_registers: null argument");
+  // a tendency evaluation of the current state that keeps the lag state and writes no dSdt
+  // register of the caller's (the model's own buffer takes it)
+  double *scratch = nullptr;
+  const size_t n = (size_t)(m->N + 1) * (m->N + 1) * m->nVar * m->nEl;
+  SELFAMD_HIP(hipMalloc(&scratch, sizeof(double) * n));
+  hipError_t e = m->launch_stage(1, 0, 0.0, m->cur, scratch, -1, flux, fluxDivergence);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  (void)hipFree(scratch);
+  SELFAMD_HIP(e);
   return kOk;
 }
 

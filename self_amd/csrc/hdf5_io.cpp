@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <string>
 
+#include <vector>
+
 #include "../../include/self_amd_io.h"
 
 namespace {
@@ -157,6 +159,34 @@ int selfamd_pickup_write(const char *path, int N, int nVar, int nEl, const doubl
             write_double(f.id, "/state/interior/solution", 4, dSol, solution) &&
             write_double(f.id, "/state/boundary/solution", 4, dSolB, solutionBoundary) &&
             write_double(f.id, "/mesh/interior/x", 5, dX, x) && write_double(f.id, "/mesh/boundary/x", 5, dXB, xBoundary);
+  if (!ok) return fail(kFileError, "%s: writing %s failed", who, path);
+  return kOk;
+}
+
+int selfamd_pickup_write_registers(const char *path, int N, int nVar, int nEl, const double *fluxDivergence,
+                                   const double *flux, const double *fluxBoundary, const double *solutionGradient,
+                                   const double *solutionGradientBoundary) {
+  const char *who = "pickup_write_registers";
+  if (!path) return fail(kBadArgument, "%s: null path", who);
+  if (N < 1 || nVar < 1 || nEl < 1) return fail(kBadArgument, "%s: bad sizes N=%d nVar=%d nEl=%d", who, N, nVar, nEl);
+  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  H5Handle f(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), H5Fclose);
+  if (!f.ok()) return fail(kFileError, "%s: cannot open %s", who, path);
+  const hsize_t P = N + 1, V = nVar, E = nEl;
+  const hsize_t dSol[4] = {E, V, P, P}, dVec[5] = {E, V, P, P, 2}, dVecB[5] = {E, 4, V, P, 2};
+  const size_t nVecB = (size_t)E * 4 * V * P * 2, nVec = (size_t)E * V * P * P * 2;
+  std::vector<double> zeros;
+  auto or_zeros = [&](const double *a, size_t n) {
+    if (a) return a;
+    if (zeros.size() < n) zeros.assign(n, 0.0);
+    return (const double *)zeros.data();
+  };
+  bool ok = write_double(f.id, "/state/interior/fluxDivergence", 4, dSol, or_zeros(fluxDivergence, nVec / 2));
+  ok = ok && write_double(f.id, "/state/interior/flux", 5, dVec, or_zeros(flux, nVec));
+  ok = ok && write_double(f.id, "/state/boundary/flux", 5, dVecB, or_zeros(fluxBoundary, nVecB));
+  ok = ok && write_double(f.id, "/state/interior/solutionGradient", 5, dVec, or_zeros(solutionGradient, nVec));
+  ok = ok && write_double(f.id, "/state/boundary/solutionGradient", 5, dVecB,
+                          or_zeros(solutionGradientBoundary, nVecB));
   if (!ok) return fail(kFileError, "%s: writing %s failed", who, path);
   return kOk;
 }

@@ -47,6 +47,13 @@ MODULE SELF_IO
       INTEGER(C_INT), VALUE :: N, nVar, nEl
       REAL(C_DOUBLE) :: xi(*), w(*), dgM(*), dM(*), bM(*), sol(*), solB(*), x(*), xb(*)
     END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_pickup_write_registers(path, N, nVar, nEl, fluxDiv, flux, fluxB, grad, gradB) &
+        BIND(C, name="selfamd_pickup_write_registers")
+      IMPORT :: C_INT, C_CHAR, C_PTR
+      CHARACTER(KIND=C_CHAR) :: path(*)
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+      TYPE(C_PTR), VALUE :: fluxDiv, flux, fluxB, grad, gradB  ! host arrays; C_NULL_PTR: zeros
+    END FUNCTION
     INTEGER(C_INT) FUNCTION selfamd_pickup_read(path, N, nVar, nEl, sol) BIND(C, name="selfamd_pickup_read")
       IMPORT :: C_INT, C_CHAR, C_DOUBLE
       CHARACTER(KIND=C_CHAR) :: path(*)
@@ -126,8 +133,9 @@ CONTAINS
   SUBROUTINE WriteModel(model, fileName)
     CLASS(Model2D), INTENT(inout) :: model
     CHARACTER(*), INTENT(in) :: fileName
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: x(:), xb(:)
-    INTEGER :: P, nEl
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: x(:), xb(:), fdiv(:), fl(:), flb(:)
+    TYPE(C_PTR) :: fdiv_dev, fl_dev
+    INTEGER :: P, nEl, nVar
     TYPE(Lagrange), POINTER :: interp
     interp => model % solution % interp
     P = interp % N + 1
@@ -143,6 +151,31 @@ CONTAINS
     CALL io_check(selfamd_pickup_write(cstr(fileName), INT(interp % N, C_INT), INT(model % solution % nVar, C_INT), &
                   INT(nEl, C_INT), interp % controlPoints, interp % qWeights, interp % dgMatrix, interp % dMatrix, &
                   interp % bMatrix, model % solution % interior, model % solution % boundary, x, xb), 'WriteModel')
+    ! the flux and fluxDivergence registers: on the per-operator path those of the last
+    ! tendency evaluation (as the reference's), on the fused engine those of the current state
+    nVar = model % solution % nVar
+    ALLOCATE (fdiv(P*P*nVar*nEl), fl(2*P*P*nVar*nEl), flb(2*P*nVar*4*nEl))
+    IF (model % plugins) THEN
+      fdiv_dev = model % fluxDivergence % interior_dev
+      fl_dev = model % flux % interior_dev
+      CALL hip_check(selfamd_memcpy(C_LOC(flb), model % flux % boundary_dev, INT(SIZE(flb), C_SIZE_T)*8_C_SIZE_T, &
+                     hipMemcpyDeviceToHost), 'WriteModel')
+    ELSE
+      fdiv_dev = device_alloc(INT(SIZE(fdiv), C_SIZE_T)*8_C_SIZE_T)
+      fl_dev = device_alloc(INT(SIZE(fl), C_SIZE_T)*8_C_SIZE_T)
+      CALL selfamd_check(selfamd_model2d_flux_registers(model % engine, fdiv_dev, fl_dev), 'WriteModel')
+      flb = 0.0_prec
+    END IF
+    CALL hip_check(selfamd_memcpy(C_LOC(fdiv), fdiv_dev, INT(SIZE(fdiv), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
+                   'WriteModel')
+    CALL hip_check(selfamd_memcpy(C_LOC(fl), fl_dev, INT(SIZE(fl), C_SIZE_T)*8_C_SIZE_T, hipMemcpyDeviceToHost), &
+                   'WriteModel')
+    IF (.NOT. model % plugins) THEN
+      CALL device_free(fdiv_dev)
+      CALL device_free(fl_dev)
+    END IF
+    CALL io_check(selfamd_pickup_write_registers(cstr(fileName), INT(interp % N, C_INT), INT(nVar, C_INT), &
+                  INT(nEl, C_INT), C_LOC(fdiv), C_LOC(fl), C_LOC(flb), C_NULL_PTR, C_NULL_PTR), 'WriteModel')
   END SUBROUTINE WriteModel
 
   ! Read_Model2D (src/SELF_Model2D.f90:1061-1099): the solution of a pickup file (its N must

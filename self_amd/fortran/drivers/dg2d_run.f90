@@ -12,24 +12,67 @@
 ! Outputs: <prefix>_tend.bin (tendency of the initial state) and <prefix>_out.bin
 ! (solution after nSteps RK3 steps).  With Mtarget >= 1 in the namelist the interpolant's target
 ! grid is Mtarget+1 uniform points and the final state is also written by WriteTecplot to <prefix>.tec.
+! With plugins = .true. the model is a user type extension of the physics (module
+! user_models below), so CalculateTendency and ForwardStep take the per-operator path through
+! the plug-in methods; massSource /= 0 selects the extension of ShallowWater whose own
+! SourceMethod adds that constant to the thickness tendency.
+MODULE user_models
+  USE SELF_Constants
+  USE SELF_Model2D
+  IMPLICIT NONE
+
+  ! extensions that override nothing: the physics' own plug-in methods, per operator
+  TYPE, EXTENDS(ShallowWater) :: UserShallowWater
+  END TYPE UserShallowWater
+  TYPE, EXTENDS(LinearShallowWater) :: UserLinearShallowWater
+  END TYPE UserLinearShallowWater
+  TYPE, EXTENDS(CompressibleIdealGas2D) :: UserCompressibleIdealGas2D
+  END TYPE UserCompressibleIdealGas2D
+
+  ! a user's own source method, written on host arrays: the parent's source plus a constant
+  ! mass source
+  TYPE, EXTENDS(ShallowWater) :: ForcedShallowWater
+    REAL(prec) :: massSource = 0.0_prec
+  CONTAINS
+    PROCEDURE :: SourceMethod => Source_ForcedShallowWater
+  END TYPE ForcedShallowWater
+
+CONTAINS
+
+  SUBROUTINE Source_ForcedShallowWater(this)
+    CLASS(ForcedShallowWater), INTENT(inout) :: this
+    CALL this % ShallowWater % SourceMethod()
+    CALL this % source % UpdateHost()
+    this % source % interior(:,:,3,:) = this % source % interior(:,:,3,:) + this % massSource
+    CALL this % source % UpdateDevice()
+  END SUBROUTINE Source_ForcedShallowWater
+
+END MODULE user_models
+
 PROGRAM dg2d_run
   USE SELF_Constants
   USE SELF_Lagrange
   USE SELF_Mesh
   USE SELF_Model2D
   USE SELF_IO
+  USE user_models
   IMPLICIT NONE
   INTEGER :: model = 2, N = 7, Mtarget = -1, nx = 4, ny = 4, bcid = SELF_BC_NONORMALFLOW, nSteps = 10
-  LOGICAL :: periodic = .FALSE., nodal = .FALSE.
-  REAL(prec) :: dt = 1.0e-4_prec
+  LOGICAL :: periodic = .FALSE., nodal = .FALSE., plugins = .FALSE.
+  REAL(prec) :: dt = 1.0e-4_prec, massSource = 0.0_prec
   CHARACTER(LEN=400) :: prefix = 'dg2d', meshFile = '', parfile
-  NAMELIST /run/ model, N, Mtarget, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix, meshFile
+  NAMELIST /run/ model, N, Mtarget, nx, ny, periodic, bcid, nSteps, dt, nodal, prefix, meshFile, plugins, &
+    massSource
   TYPE(Lagrange), TARGET :: interp
   TYPE(Mesh2D), TARGET :: mesh
   TYPE(SEMQuad), TARGET :: geometry
   TYPE(ShallowWater) :: sw
   TYPE(LinearShallowWater) :: lsw
   TYPE(CompressibleIdealGas2D) :: euler
+  TYPE(UserShallowWater) :: usw
+  TYPE(UserLinearShallowWater) :: ulsw
+  TYPE(UserCompressibleIdealGas2D) :: ueuler
+  TYPE(ForcedShallowWater) :: fsw
   REAL(prec), ALLOCATABLE :: sol(:,:,:,:), f1(:,:,:), f2(:,:,:)
   INTEGER :: u, nVar
 
@@ -58,28 +101,57 @@ PROGRAM dg2d_run
   CALL read_array(TRIM(prefix)//'_field1.bin', f1, SIZE(f1))
   CALL read_array(TRIM(prefix)//'_field2.bin', f2, SIZE(f2))
 
-  SELECT CASE (model)
-  CASE (1)
-    CALL lsw % Init(nVar, mesh, geometry)
-    CALL lsw % SetNodalMetrics(nodal)
-    CALL lsw % SetBathymetry(f1)
-    CALL lsw % SetCoriolis(f2)
-    CALL run_model(lsw)
-  CASE (2)
-    CALL sw % Init(nVar, mesh, geometry)
-    CALL sw % SetNodalMetrics(nodal)
-    CALL sw % SetTopography(f1)
-    CALL run_model(sw)
-  CASE (3)
-    CALL euler % Init(nVar, mesh, geometry)
-    CALL euler % SetNodalMetrics(nodal)
-    CALL euler % SetGravity(f1)
-    CALL euler % SetDrag(f2)
-    CALL run_model(euler)
-  CASE DEFAULT
-    PRINT *, 'dg2d_run: unknown model ', model
-    ERROR STOP 2
-  END SELECT
+  IF (plugins) THEN
+    SELECT CASE (model)
+    CASE (1)
+      CALL ulsw % Init(nVar, mesh, geometry)
+      CALL ulsw % SetBathymetry(f1)
+      CALL ulsw % SetCoriolis(f2)
+      CALL run_model(ulsw)
+    CASE (2)
+      IF (massSource /= 0.0_prec) THEN
+        fsw % massSource = massSource
+        CALL fsw % Init(nVar, mesh, geometry)
+        CALL fsw % SetTopography(f1)
+        CALL run_model(fsw)
+      ELSE
+        CALL usw % Init(nVar, mesh, geometry)
+        CALL usw % SetTopography(f1)
+        CALL run_model(usw)
+      END IF
+    CASE (3)
+      CALL ueuler % Init(nVar, mesh, geometry)
+      CALL ueuler % SetGravity(f1)
+      CALL ueuler % SetDrag(f2)
+      CALL run_model(ueuler)
+    CASE DEFAULT
+      PRINT *, 'dg2d_run: unknown model ', model
+      ERROR STOP 2
+    END SELECT
+  ELSE
+    SELECT CASE (model)
+    CASE (1)
+      CALL lsw % Init(nVar, mesh, geometry)
+      CALL lsw % SetNodalMetrics(nodal)
+      CALL lsw % SetBathymetry(f1)
+      CALL lsw % SetCoriolis(f2)
+      CALL run_model(lsw)
+    CASE (2)
+      CALL sw % Init(nVar, mesh, geometry)
+      CALL sw % SetNodalMetrics(nodal)
+      CALL sw % SetTopography(f1)
+      CALL run_model(sw)
+    CASE (3)
+      CALL euler % Init(nVar, mesh, geometry)
+      CALL euler % SetNodalMetrics(nodal)
+      CALL euler % SetGravity(f1)
+      CALL euler % SetDrag(f2)
+      CALL run_model(euler)
+    CASE DEFAULT
+      PRINT *, 'dg2d_run: unknown model ', model
+      ERROR STOP 2
+    END SELECT
+  END IF
   PRINT '(A,I0,A,I0,A)', 'dg2d_run: model ', model, ' done (', nSteps, ' RK3 steps)'
 
 CONTAINS
@@ -108,6 +180,7 @@ CONTAINS
     CALL ReadModel(m, TRIM(prefix)//'_pickup.h5')
     CALL m % UpdateHost()
     CALL write_array(TRIM(prefix)//'_reread.bin', m % solution % interior, SIZE(m % solution % interior))
+    IF (m % plugins) PRINT '(A)', 'dg2d_run: per-operator path (plug-in methods)'
     CALL m % Free()
   END SUBROUTINE run_model
 

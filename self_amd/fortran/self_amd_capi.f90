@@ -196,6 +196,11 @@ MODULE SELF_AMD_CAPI
       INTEGER(C_INT), VALUE :: nSteps
       REAL(C_DOUBLE), VALUE :: dt
     END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_model2d_flux_registers(model, fluxDiv, flux) &
+        BIND(C, name="selfamd_model2d_flux_registers")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: model, fluxDiv, flux
+    END FUNCTION
     INTEGER(C_INT) FUNCTION selfamd_model2d_synchronize(model) BIND(C, name="selfamd_model2d_synchronize")
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: model
@@ -204,6 +209,125 @@ MODULE SELF_AMD_CAPI
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: model
       REAL(C_DOUBLE), VALUE :: t
+    END FUNCTION
+
+    ! ---- register updates (the per-operator path of CalculateTendency / LowStorageRK3) ----
+    INTEGER(C_INT) FUNCTION selfamd_update_grk(workSol, S, dSdt, a, g, dt, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_update_grk")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: workSol, S, dSdt, stream
+      REAL(C_DOUBLE), VALUE :: a, g, dt
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_calculate_dsdt(fluxDiv, source, dSdt, N, nVar, nEl, stream) &
+        BIND(C, name="selfamd_calculate_dsdt")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: fluxDiv, source, dSdt, stream
+      INTEGER(C_INT), VALUE :: N, nVar, nEl
+    END FUNCTION
+
+    ! ---- physics operators, one per plug-in method ----
+    INTEGER(C_INT) FUNCTION selfamd_sw2d_velocity(sol, solB, vel, velB, N, nEl, stream) &
+        BIND(C, name="selfamd_sw2d_velocity")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: sol, solB, vel, velB, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_sw2d_set_boundary_condition(solB, solE, topoB, nHatb, sideInfo, N, nEl, stream) &
+        BIND(C, name="selfamd_sw2d_set_boundary_condition")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: solB, solE, topoB, nHatb, sideInfo, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_sw2d_source(source, sol, gradTopo, g, fCori, N, nEl, stream) &
+        BIND(C, name="selfamd_sw2d_source")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: source, sol, gradTopo, stream
+      REAL(C_DOUBLE), VALUE :: g, fCori
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_sw2d_flux(flux, sol, vel, g, N, nEl, stream) BIND(C, name="selfamd_sw2d_flux")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: flux, sol, vel, stream
+      REAL(C_DOUBLE), VALUE :: g
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_sw2d_riemann(fluxBN, solB, solE, velB, velE, nHatb, nScaleb, g, N, nEl, stream) &
+        BIND(C, name="selfamd_sw2d_riemann")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: fluxBN, solB, solE, velB, velE, nHatb, nScaleb, stream
+      REAL(C_DOUBLE), VALUE :: g
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_lsw2d_set_boundary_condition(solB, solE, nHatb, sideInfo, N, nEl, stream) &
+        BIND(C, name="selfamd_lsw2d_set_boundary_condition")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: solB, solE, nHatb, sideInfo, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_lsw2d_source(source, sol, fCori, N, nEl, stream) BIND(C, name="selfamd_lsw2d_source")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: source, sol, fCori, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_lsw2d_flux(flux, sol, H, g, N, nEl, stream) BIND(C, name="selfamd_lsw2d_flux")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: flux, sol, H, stream
+      REAL(C_DOUBLE), VALUE :: g
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_lsw2d_riemann(fluxBN, solB, solE, Hb, nHatb, nScaleb, g, N, nEl, stream) &
+        BIND(C, name="selfamd_lsw2d_riemann")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: fluxBN, solB, solE, Hb, nHatb, nScaleb, stream
+      REAL(C_DOUBLE), VALUE :: g
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_diagnostics(sol, diag, gamma, R, N, nEl, stream) &
+        BIND(C, name="selfamd_euler2d_diagnostics")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: sol, diag, stream
+      REAL(C_DOUBLE), VALUE :: gamma, R
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_primitive(sol, prim, gamma, N, nEl, stream) &
+        BIND(C, name="selfamd_euler2d_primitive")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: sol, prim, stream
+      REAL(C_DOUBLE), VALUE :: gamma
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_entropy_vars(sol, ent, gamma, N, nEl, stream) &
+        BIND(C, name="selfamd_euler2d_entropy_vars")
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: sol, ent, stream
+      REAL(C_DOUBLE), VALUE :: gamma
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_set_boundary_condition(solB, solE, presSolB, primB, primE, presPrimB, &
+        diagB, diagE, presDiagB, nHatb, sideInfo, N, nEl, stream) BIND(C, name="selfamd_euler2d_set_boundary_condition")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: solB, solE, presSolB, primB, primE, presPrimB, diagB, diagE, presDiagB, nHatb, sideInfo
+      TYPE(C_PTR), VALUE :: stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_source(source, sol, env, envGrad, N, nEl, stream) &
+        BIND(C, name="selfamd_euler2d_source")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: source, sol, env, envGrad, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_flux(flux, sol, prim, diag, N, nEl, stream) &
+        BIND(C, name="selfamd_euler2d_flux")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: flux, sol, prim, diag, stream
+      INTEGER(C_INT), VALUE :: N, nEl
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION selfamd_euler2d_riemann(fluxBN, solB, solE, primB, primE, diagB, diagE, nHatb, nScaleb, &
+        N, nEl, stream) BIND(C, name="selfamd_euler2d_riemann")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: fluxBN, solB, solE, primB, primE, diagB, diagE, nHatb, nScaleb, stream
+      INTEGER(C_INT), VALUE :: N, nEl
     END FUNCTION
   END INTERFACE
 

@@ -53,6 +53,72 @@ def test_fortran_driver_parity(kind, model, N, n, periodic, bcid, tmp_path):
         assert np.array_equal(sol, out)
 
 
+def _driver_inputs(c, kind, tmp_path):
+    prefix = str(tmp_path / kind)
+    c.sol.astype(np.float64).tofile(prefix + "_sol.bin")
+    f1, f2 = {"sw": ("topo", None), "lsw": ("H", "f"), "euler": ("phi", "drag")}[kind]
+    np.ascontiguousarray(c.fields[f1]).tofile(prefix + "_field1.bin")
+    (np.ascontiguousarray(c.fields[f2]) if f2 else np.zeros_like(c.fields[f1])).tofile(prefix + "_field2.bin")
+    return prefix
+
+
+@pytest.mark.parametrize("kind,model,N,n,periodic,bcid", [("sw", 2, 7, 5, True, 102), ("sw", 2, 5, 4, False, 102),
+                                                        ("lsw", 1, 5, 5, False, 102), ("euler", 3, 4, 4, False, 102),
+                                                        ("euler", 3, 3, 4, False, 100)])
+def test_fortran_plugin_path_parity(kind, model, N, n, periodic, bcid, tmp_path):
+    """A user type extension of each physics (overriding nothing) runs the per-operator path:
+    CalculateTendency_Model2D's sequence of PreTendency, MappedScalar2D%BoundaryInterp /
+    SideExchange, SetBoundaryCondition, SourceMethod, RiemannSolver, FluxMethod,
+    MappedVector2D%ContravariantProjection / Divergence (VectorDGDivergence_2D,
+    JacobianWeight), and UpdateGRK3 in the reference's RK3 loop, with the SEMQuad per-node
+    metric terms.  Tendency and RK3 steps match the unmodified oracle."""
+    assert os.path.exists(DRIVER), "build the Fortran layer first (make -C self_amd/fortran)"
+    c = Case(kind, N, n, n, periodic=periodic, bcid=bcid, seed=12, metrics="nodal")
+    prefix = _driver_inputs(c, kind, tmp_path)
+    dt, steps = c.dt(), 5
+    par = tmp_path / "run.nml"
+    par.write_text(f"&run model={model}, N={N}, nx={n}, ny={n}, periodic={'.true.' if periodic else '.false.'}, "
+                   f"bcid={bcid}, nSteps={steps}, dt={float(dt)!r}, plugins=.true., prefix='{prefix}' /\n")
+    r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "per-operator path" in r.stdout
+    shape = c.sol.shape
+    tend = np.fromfile(prefix + "_tend.bin").reshape(shape)
+    out = np.fromfile(prefix + "_out.bin").reshape(shape)
+    assert rel_err(tend, c.om.tendency()).max() <= TOL
+    entropy, mass = np.loadtxt(prefix + "_diag.txt")
+    assert abs(mass - c.om.mass(2)) <= TOL * abs(c.om.mass(2))
+    c.om.rk3_steps(steps, dt)
+    assert rel_err(out, c.om.solution).max() <= TOL
+
+
+def test_fortran_user_source_method(tmp_path):
+    """A user's ShallowWater extension with its own SourceMethod (the parent's source, then a
+    constant mass source added on host arrays): its tendency is the oracle's plus that
+    constant in the thickness equation, and over RK3 steps the mass grows by exactly
+    massSource * area * t (the periodic mesh conserves the rest)."""
+    assert os.path.exists(DRIVER), "build the Fortran layer first (make -C self_amd/fortran)"
+    N, n, s0 = 5, 6, 0.01
+    c = Case("sw", N, n, n, periodic=True, seed=13, metrics="nodal")
+    prefix = _driver_inputs(c, "sw", tmp_path)
+    dt, steps = c.dt(), 6
+    par = tmp_path / "run.nml"
+    par.write_text(f"&run model=2, N={N}, nx={n}, ny={n}, periodic=.true., bcid=0, nSteps={steps}, "
+                   f"dt={float(dt)!r}, plugins=.true., massSource={s0!r}, prefix='{prefix}' /\n")
+    r = subprocess.run([DRIVER, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    shape = c.sol.shape
+    tend = np.fromfile(prefix + "_tend.bin").reshape(shape)
+    want = c.om.tendency()
+    want[:, 2] += s0
+    assert rel_err(tend, want).max() <= TOL
+    out = np.fromfile(prefix + "_out.bin").reshape(shape)
+    w = c.weights
+    mass = lambda s: float(np.sum(s[:, 2] * c.geom.J * w[None, :, None] * w[None, None, :]))
+    m0 = mass(c.sol)
+    assert abs(mass(out) - (m0 + s0 * steps * dt)) <= 1e-13 * m0
+
+
 def test_fortran_driver_on_hopr_mesh(tmp_path):
     """Read_HOPr of the Fortran SELF_IO module: the driver reads the Block2D HOPr mesh file
     (committed copy of the reference's etc/mesh/Block2D/Block2D_mesh.h5), resets its boundary

@@ -1,6 +1,8 @@
 """Fused-engine parity on the GPU: CalculateTendency and low-storage RK3 trajectories of
 the LSW / SW / Euler models against the oracle on identical seeded inputs (relative
 max-norm per variable <= 1e-12), plus full-size checks (SW 256^2 and Euler 512^2, N = 7)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -123,6 +125,23 @@ def test_multi_step_graph_matches_single_steps(kind):
     assert a.t == b.t
     c.om.rk3_steps(19, dt)
     assert rel_err(a.solution, c.om.solution).max() <= TOL
+
+
+@pytest.mark.parametrize("kind,metrics", [("sw", "nodal"), ("lsw", "affine"), ("euler", "nodal"), ("sw", "affine")])
+def test_flux_registers_match_oracle(kind, metrics):
+    """The flux (after ContravariantProjection) and fluxDivergence (after JacobianWeight)
+    registers of CalculateTendency_Model2D for the current state, against the oracle's; the
+    evaluation leaves the state and the SW lag state alone (the following steps still match)."""
+    c = Case(kind, 5, 4, 4, periodic=(kind == "sw"), seed=3, metrics=metrics)
+    m, _ = c.gpu_model()
+    fdiv, fl = m.flux_registers()
+    c.om.tendency()
+    assert rel_err(fdiv, c.om.field("fluxDiv", c.om.interior_shape)).max() <= TOL
+    assert rel_err(fl, c.om.field("flux", c.om.interior_shape + (2,))).max() <= TOL
+    dt = c.dt()
+    m.ForwardStep(3, dt)
+    c.om.rk3_steps(3, dt)
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
 
 
 @pytest.mark.parametrize("bands", [0, 3])
@@ -398,6 +417,13 @@ def test_pickup_round_trip_and_resume(kind, N, tmp_path):
     a.ForwardStep(3, dt)
     path = str(tmp_path / "solution.pickup.h5")
     a.WriteModel(path)
+    # the flux and fluxDivergence registers in the file are those of the current state
+    import ctypes
+    fdiv, fl = a.flux_registers()
+    for name, want in (("/state/interior/fluxDivergence", fdiv), ("/state/interior/flux", fl)):
+        got = np.empty(want.shape)
+        _lib.call_io("selfamd_hopr_read_double", os.fsencode(path), name.encode(), got.ctypes.data, got.size)
+        assert np.array_equal(got, want), name
     b = model()
     b.ReadModel(path)
     assert np.array_equal(b.solution, a.solution)

@@ -51,6 +51,35 @@ def test_layout_and_round_trip(tmp_path):
     assert np.array_equal(xb, arrays[8])
 
 
+def test_register_datasets(tmp_path):
+    """The rest of Write_Model2D's datasets (src/SELF_Model2D.f90:981-990): fluxDivergence,
+    flux and solutionGradient (interior and boundary), in the reference's layout; a null array
+    is written as zeros."""
+    rng = np.random.default_rng(7)
+    N, nVar, nEl = 4, 3, 6
+    P = N + 1
+    path = os.fsencode(str(tmp_path / "regs.h5"))
+    write(path, N, nVar, nEl, rng)
+    fdiv = rng.normal(size=(nEl, nVar, P, P))
+    fl = rng.normal(size=(nEl, nVar, P, P, 2))
+    flb = rng.normal(size=(nEl, 4, nVar, P, 2))
+    _lib.call_io("selfamd_pickup_write_registers", path, N, nVar, nEl, fdiv.ctypes.data, fl.ctypes.data,
+                 flb.ctypes.data, None, None)
+    assert dims(path, "/state/interior/fluxDivergence") == (nEl, nVar, P, P)
+    assert dims(path, "/state/interior/flux") == (nEl, nVar, P, P, 2)
+    assert dims(path, "/state/boundary/flux") == (nEl, 4, nVar, P, 2)
+    assert dims(path, "/state/interior/solutionGradient") == (nEl, nVar, P, P, 2)
+    assert dims(path, "/state/boundary/solutionGradient") == (nEl, 4, nVar, P, 2)
+    for name, want in (("/state/interior/fluxDivergence", fdiv), ("/state/interior/flux", fl),
+                       ("/state/boundary/flux", flb), ("/state/interior/solutionGradient", np.zeros_like(fl))):
+        got = np.empty(want.shape)
+        _lib.call_io("selfamd_hopr_read_double", path, name.encode(), got.ctypes.data, got.size)
+        assert np.array_equal(got, want), name
+    # the solution is still there, unchanged
+    sol = np.empty((nEl, nVar, P, P))
+    _lib.call_io("selfamd_pickup_read", path, N, nVar, nEl, sol.ctypes.data)
+
+
 def test_read_rejects_mismatches(tmp_path):
     rng = np.random.default_rng(6)
     path = os.fsencode(str(tmp_path / "p.h5"))
