@@ -304,8 +304,13 @@ int selfamd_model2d_integral(selfamd_model2d *model, int var, double *value);
 int selfamd_model2d_forward_step(selfamd_model2d *model, int nSteps, double dt);
 /* Blocks until the model's stream has drained. */
 int selfamd_model2d_synchronize(selfamd_model2d *model);
+/* The HIP stream (hipStream_t, as void*) that the model's kernels and graphs run on, so that
+ * a caller can order its own work with the model's.  No reference counterpart (the reference
+ * launches everything on the default stream). */
+int selfamd_model2d_stream(selfamd_model2d *model, void **stream);
 /* forward_step bracketed by two hipEvents on the model's stream; returns the elapsed
- * milliseconds between them (synchronises). */
+ * milliseconds between them (synchronises).  Graphs that the steps need are captured before
+ * the first event. */
 int selfamd_model2d_time_steps(selfamd_model2d *model, int nSteps, double dt, double *elapsedMs);
 /* Tile variants of the stage kernel (affine metrics) of nonlinear SW at N = 3, 5, 7 and of
  * LSW and Euler at N = 7: block size (64, 128, 256 work-items), element order alternating

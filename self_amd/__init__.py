@@ -454,6 +454,14 @@ class Model2D:
         call("selfamd_model2d_time_steps", self._h, int(nSteps), float(dt), ctypes.byref(ms))
         return ms.value
 
+    @property
+    def stream(self):
+        """The model's HIP stream handle (an integer address), for ordering other work with
+        the model's (e.g. the library's copy kernel)."""
+        st = ctypes.c_void_p()
+        call("selfamd_model2d_stream", self._h, ctypes.byref(st))
+        return st.value
+
     def solution_device(self):
         """Device copy of the solution (a buffer owned by the model, overwritten by the next
         solution/tendency call)."""

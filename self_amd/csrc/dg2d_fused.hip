@@ -1429,23 +1429,39 @@ int selfamd_model2d_flux_registers(selfamd_model2d *m, double *fluxDivergence, d
   return kOk;
 }
 
-int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
-  SELFAMD_REQUIRE(m && nSteps >= 0 && dt > 0.0, "model2d_forward_step: bad arguments (nSteps=%d dt=%g)", nSteps, dt);
-  if (nSteps == 0) return kOk;
-  if (m->graphDt > 0.0 && m->graphDt != dt && m->graphEpoch == g_tileEpoch && nSteps <= kDirectSteps) {
-    // a few steps of another size (e.g. ForwardStep's shorter last step up to tn): launched
-    // directly, so the graphs of the running step size survive
-    SELFAMD_HIP(m->launch_steps(nSteps, dt));
-    return kOk;
-  }
+namespace {
+// a few steps of another size than the captured graphs' (e.g. ForwardStep's shorter last step
+// up to tn) are launched directly, so the graphs of the running step size survive
+bool direct_steps(const selfamd_model2d *m, int nSteps, double dt) {
+  return m->graphDt > 0.0 && m->graphDt != dt && m->graphEpoch == g_tileEpoch && nSteps <= kDirectSteps;
+}
+// captures the graphs that nSteps steps of size dt from the current trace buffer replay
+hipError_t prepare_graphs(selfamd_model2d *m, int nSteps, double dt) {
+  if (nSteps == 0 || direct_steps(m, nSteps, dt)) return hipSuccess;
   if (m->graphDt != dt || m->graphEpoch != g_tileEpoch) {
     m->drop_graphs();
-    for (int start = 0; start < 2; start++) SELFAMD_HIP(m->capture(start, 1, dt, &m->graph[start]));
+    for (int start = 0; start < 2; start++) {
+      hipError_t e = m->capture(start, 1, dt, &m->graph[start]);
+      if (e != hipSuccess) return e;
+    }
     m->graphDt = dt;
     m->graphEpoch = g_tileEpoch;
   }
   const int K = selfamd_model2d::kGraphSteps;
-  if (nSteps >= K && !m->graphK[m->cur]) SELFAMD_HIP(m->capture(m->cur, K, dt, &m->graphK[m->cur]));
+  if (nSteps >= K && !m->graphK[m->cur]) return m->capture(m->cur, K, dt, &m->graphK[m->cur]);
+  return hipSuccess;
+}
+}  // namespace
+
+int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
+  SELFAMD_REQUIRE(m && nSteps >= 0 && dt > 0.0, "model2d_forward_step: bad arguments (nSteps=%d dt=%g)", nSteps, dt);
+  if (nSteps == 0) return kOk;
+  if (direct_steps(m, nSteps, dt)) {
+    SELFAMD_HIP(m->launch_steps(nSteps, dt));
+    return kOk;
+  }
+  SELFAMD_HIP(prepare_graphs(m, nSteps, dt));
+  const int K = selfamd_model2d::kGraphSteps;
   int n = 0;
   for (; n + K <= nSteps; n += K) {
     SELFAMD_HIP(hipGraphLaunch(m->graphK[m->cur], m->stream));  // 3K stage flips: parity unchanged
@@ -1465,8 +1481,17 @@ int selfamd_model2d_synchronize(selfamd_model2d *m) {
   return kOk;
 }
 
+int selfamd_model2d_stream(selfamd_model2d *m, void **stream) {
+  SELFAMD_REQUIRE(m && stream, "model2d_stream: null argument");
+  *stream = (void *)m->stream;
+  return kOk;
+}
+
 int selfamd_model2d_time_steps(selfamd_model2d *m, int nSteps, double dt, double *elapsedMs) {
-  SELFAMD_REQUIRE(m && elapsedMs, "model2d_time_steps: null argument");
+  SELFAMD_REQUIRE(m && elapsedMs && nSteps >= 0 && dt > 0.0, "model2d_time_steps: bad arguments");
+  // graph capture is host work: done before the first event, so the GPU's idle time during it
+  // is not timed
+  SELFAMD_HIP(prepare_graphs(m, nSteps, dt));
   hipEvent_t e0, e1;
   SELFAMD_HIP(hipEventCreate(&e0));
   SELFAMD_HIP(hipEventCreate(&e1));
