@@ -300,6 +300,16 @@ int selfamd_model2d_entropy(selfamd_model2d *model, double *entropy);
  * the pattern of Integral_MappedScalar2D (src/SELF_MappedData.f90:1270-1311) with the
  * variable as an argument. */
 int selfamd_model2d_integral(selfamd_model2d *model, int var, double *value);
+/* selfamd_model2d_integral without any allocation: the same kernels and the same result, bit
+ * for bit, written to *value_dev (device memory) on the model's stream, without
+ * synchronising.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums live in a 512 KiB device array of the library, one per GPU,
+ * kept until the process ends; meshes of more than 16,777,216 nodes ((N+1)^2 nEl) are
+ * rejected with an error.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (other streams, other host threads) race on it.  The current device must be the
+ * model's. */
+int selfamd_model2d_integral_noalloc(selfamd_model2d *model, int var, double *value_dev);
 /* nSteps full low-storage RK3 steps of size dt (ForwardStep with tn = t + nSteps*dt). */
 int selfamd_model2d_forward_step(selfamd_model2d *model, int nSteps, double dt);
 /* Blocks until the model's stream has drained. */

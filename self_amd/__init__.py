@@ -348,6 +348,11 @@ class Model2D:
         call("selfamd_model2d_integral", self._h, int(var), ctypes.byref(v))
         return v.value
 
+    def IntegralNoAlloc(self, var, out):
+        """Integral(var) written to the device double `out` (a DeviceArray) on the model's
+        stream by the library's no-allocation entry point; does not synchronise."""
+        call("selfamd_model2d_integral_noalloc", self._h, int(var), out.ptr)
+
     def WriteModel(self, path):
         """WriteModel (Write_Model2D, src/SELF_Model2D.f90:861-1059) to an HDF5 pickup file:
         quadrature, interior and boundary solution, interior and boundary mesh positions, and

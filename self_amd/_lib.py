@@ -85,6 +85,7 @@ SIGNATURES = {
     "selfamd_model2d_synchronize": [c_void_p],
     "selfamd_model2d_time_steps": [c_void_p, c_int, c_double, c_void_p],
     "selfamd_model2d_stream": [c_void_p, c_void_p],
+    "selfamd_model2d_integral_noalloc": [c_void_p, c_int, c_void_p],
     "selfamd_model2d_get_time": [c_void_p, c_void_p],
     "selfamd_model2d_set_time": [c_void_p, c_double],
     "selfamd_batch_create": [c_void_p, c_int, c_void_p, c_void_p],

@@ -22,6 +22,7 @@
 // Geometry: the fast path takes affine elements (metric terms constant per element, true
 // for the structured and block meshes); each element carries one 20-double record.
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -970,6 +971,7 @@ hipError_t launch_un_init(int N, const double *tr, double *un, const double *geo
 // ------------------------------------------------------------------------------
 struct selfamd_model2d {
   int kind = 0, N = 0, nEl = 0, nVar = 0, nT = 0;
+  int device = 0;  // the GPU the model's buffers live on (current device at creation)
   hipStream_t stream = nullptr;
   bool ownStream = false;
   double *S = nullptr, *G = nullptr, *trace[2] = {nullptr, nullptr}, *un[2] = {nullptr, nullptr};
@@ -1217,6 +1219,10 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   m->p.auxConst = 0.0;  // static fields default to zero until a Set* call provides them
   m->p.stat2Const = 0.0;
   auto fail = [&](int code) { selfamd_model2d_destroy(m); return code; };
+  if (hipGetDevice(&m->device) != hipSuccess) {
+    set_error("model2d_create: hipGetDevice failed");
+    return fail(kHipError);
+  }
   if (stream) {
     m->stream = reinterpret_cast<hipStream_t>(stream);
   } else {
@@ -1386,6 +1392,58 @@ static int model_integral(selfamd_model2d *m, int what, int var, double *value, 
   SELFAMD_CHECK_LAUNCH();
   SELFAMD_HIP(hipMemcpyAsync(value, m->intScratch + nBlocks, sizeof(double), hipMemcpyDeviceToHost, m->stream));
   SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+namespace {
+// Partial sums of selfamd_model2d_integral_noalloc: one per block of kIntBlock nodes, then the
+// total.  A file-scope device array, so that entry point allocates nothing; every GPU that runs
+// it holds its own copy (512 KiB) until the process ends.  Host code must not pass `scratch`
+// itself to a kernel (in host code the name is a host-side stand-in): it uses the device
+// address that scratch_address() looks up once per device.
+constexpr int kNoAllocPartials = 65536;  // meshes of up to 65536 * 256 = 16,777,216 nodes
+__device__ __align__(256) double scratch[kNoAllocPartials + 1];
+constexpr int kMaxDevices = 64;
+double *g_scratchDev[kMaxDevices] = {};
+std::mutex g_scratchMutex;
+
+// device address of `scratch` on device `dev`, which must be the current device
+hipError_t scratch_address(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchDev[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchDev[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratchDev[dev];
+  return hipSuccess;
+}
+}  // namespace
+
+int selfamd_model2d_integral_noalloc(selfamd_model2d *m, int var, double *value) {
+  SELFAMD_REQUIRE(m && value, "model2d_integral_noalloc: null argument");
+  SELFAMD_REQUIRE(var >= 0 && var < m->nVar, "model2d_integral_noalloc: variable %d out of range (nVar = %d)", var,
+                  m->nVar);
+  const size_t PP = (size_t)(m->N + 1) * (m->N + 1), nNodes = PP * m->nEl;
+  const size_t nBlocks = (nNodes + kIntBlock - 1) / kIntBlock;
+  SELFAMD_REQUIRE(nBlocks <= (size_t)kNoAllocPartials,
+                  "model2d_integral_noalloc: the mesh has %zu nodes; this entry point takes at most %zu "
+                  "(use selfamd_model2d_integral)", nNodes, (size_t)kNoAllocPartials * kIntBlock);
+  int dev = -1;
+  SELFAMD_HIP(hipGetDevice(&dev));
+  SELFAMD_REQUIRE(dev == m->device, "model2d_integral_noalloc: the model lives on device %d, the current device is %d",
+                  m->device, dev);
+  SELFAMD_REQUIRE(dev >= 0 && dev < kMaxDevices, "model2d_integral_noalloc: device %d out of range", dev);
+  double *part = nullptr;
+  SELFAMD_HIP(scratch_address(dev, &part));
+  // every block writes its own partial sum, so nothing in the array needs clearing first
+  const double *qW = m->mats + 2 * PP + 2 * (m->N + 1);
+  hipLaunchKernelGGL((k_integral_partial<kIntVariable>), dim3((unsigned)nBlocks), dim3(kIntBlock), 0, m->stream, m->S,
+                     m->aux, m->nJ, qW, m->p, m->N, m->nVar, var, nNodes, part);
+  SELFAMD_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kIntBlock), 0, m->stream, part, (int)nBlocks, value);
+  SELFAMD_CHECK_LAUNCH();
   return kOk;
 }
 
