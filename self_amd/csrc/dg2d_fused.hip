@@ -82,6 +82,7 @@ struct FusedArgs {
   // divided by J (0:N,0:N,nVar,nEl), the flux and fluxDivergence registers of
   // CalculateTendency_Model2D (src/SELF_Model2D.f90:816-859)
   double *fluxOut, *fluxDivOut;
+  int anyBoundary;          // 1 when some element side is a physical boundary (neighbour code < 0)
   Params p;
 };
 
@@ -95,6 +96,7 @@ struct PhysSW {
   static constexpr int kKind = SELFAMD_MODEL_SW;
   static constexpr int NV = 3, NT = 3;
   static constexpr bool kAuxGrad = true, kAuxTrace = true, kLag = true;
+  static constexpr bool kAuxTraceInFlux = false;  // the topography trace enters the radiation BC only
   __device__ static void volume(const double *s, double, const Params &p, double *fx, double *fy, double *q) {
     const double Hu = s[0], Hv = s[1], H = s[2];
     const double invH = 1.0 / H;
@@ -147,6 +149,7 @@ struct PhysLSW {
   static constexpr int kKind = SELFAMD_MODEL_LSW;
   static constexpr int NV = 3, NT = 3;
   static constexpr bool kAuxGrad = false, kAuxTrace = true, kLag = false;
+  static constexpr bool kAuxTraceInFlux = true;  // the Riemann flux takes c = sqrt(g H) of the bathymetry trace
   __device__ static void volume(const double *s, double H, const Params &p, double *fx, double *fy, double *q) {
     fx[0] = p.g * s[2]; fy[0] = 0.0;
     fx[1] = 0.0;        fy[1] = p.g * s[2];
@@ -189,6 +192,7 @@ struct PhysEuler {
   static constexpr int kKind = SELFAMD_MODEL_EULER;
   static constexpr int NV = 4, NT = 9;
   static constexpr bool kAuxGrad = true, kAuxTrace = false, kLag = false;
+  static constexpr bool kAuxTraceInFlux = false;
   __device__ static void nodal_p(const double *s, const Params &p, double &u, double &v, double &pr, double &ke,
                                  double &invRho) {
     invRho = 1.0 / s[2];
@@ -271,6 +275,22 @@ template <bool NODAL>
 __device__ inline double lag_outer(const double *un, size_t fbNeighbour, double nx, double ny) {
   if constexpr (NODAL) return un[2 * fbNeighbour] * nx + un[2 * fbNeighbour + 1] * ny;
   return -un[fbNeighbour];
+}
+
+// Ordering point between kernel phases.  When every element's node lanes and face lanes lie in
+// one wavefront (WAVE), a phase only reads LDS values its own wavefront wrote, and a wavefront's
+// LDS accesses execute in program order: a code-motion barrier suffices.  A workgroup barrier
+// would also make each wavefront wait for its outstanding global stores (the workgroup-scope
+// release) and for the block's slowest wavefront.
+template <bool WAVE>
+__device__ inline void phase_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
 }
 
 __device__ inline double pair_sum32(double x) {
@@ -402,6 +422,10 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
                    : FPAD ? ((tid >> 4) & 1) + (fslot >= NP ? 2 : 0)
                           : fslotN / NP;
   const int fi = fslotN % NP, fnode = side * NP + fi;
+  // element-to-wavefront locality of the node and face lanes (see phase_sync): at N = 7 a
+  // wavefront owns one element (face lanes split in pairs); at N = 3 an element's 16 nodes and 16
+  // face nodes share the same 16 lanes
+  constexpr bool WAVE = (NN == 64 && SPLIT) || (NN == 16 && NF == 16 && ONE_FACE_PASS && !SPLIT && !FPAD);
   const int pf = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && (!FPAD || fslot < 2 * NP) && fel < EPB &&
                           pf < a.nEl;
@@ -483,8 +507,12 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     double qL[NT];
 #pragma unroll
     for (int k = 0; k < NT; k++) qL[k] = 0.0;
+    // a static field's trace that enters only boundary conditions (SW topography) is skipped on
+    // meshes without boundary sides (the condition is uniform over the launch, so the lane pairs
+    // of SPLIT stay together)
+    const bool auxTrace = Phys::kAuxTrace && a.aux && (Phys::kAuxTraceInFlux || a.anyBoundary);
     double auxB = a.p.auxConst;
-    if (Phys::kAuxTrace && a.aux) auxB = 0.0;
+    if (auxTrace) auxB = 0.0;
     if constexpr (SPLIT) {
       const int lo = fhalf ? FH0 : 0;
 #pragma unroll
@@ -494,11 +522,11 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
         const int nd = foff[t];
 #pragma unroll
         for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
-        if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
+        if (auxTrace) auxB += bm[ii] * sAux[fel_][nd];
       }
 #pragma unroll
       for (int k = 0; k < NT; k++) qL[k] = pair_sum32(qL[k]);
-      if (Phys::kAuxTrace && a.aux) auxB = pair_sum32(auxB);
+      if (auxTrace) auxB = pair_sum32(auxB);
       if (fhalf) return;  // the pair's first lane does the boundary and Riemann work
     } else {
 #pragma unroll
@@ -506,7 +534,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
         const int nd = alongJ ? slot(fi_, ii) : slot(ii, fi_);
 #pragma unroll
         for (int k = 0; k < NT; k++) qL[k] += bm[ii] * sQ[k][fel_][nd];
-        if (Phys::kAuxTrace && a.aux) auxB += bm[ii] * sAux[fel_][nd];
+        if (auxTrace) auxB += bm[ii] * sAux[fel_][nd];
       }
     }
     const bool interior = code_ >= 0;
@@ -571,7 +599,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       face(fel2, side2, fi2, fe2, fnode2, code2, qR2, unL2, unR2, nx2, ny2, nmag2);
     }
   }
-  __syncthreads();
+  phase_sync<WAVE>();
 
   // ---- phase 2: divergence, source, update ----
   if (nodeActive) {
@@ -641,7 +669,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     }
   }
   if (MODE == 1) return;
-  __syncthreads();
+  phase_sync<WAVE>();
 
   // ---- phase 3: traces of the updated state ----
   if constexpr (SPLIT) {
@@ -979,6 +1007,7 @@ struct selfamd_model2d {
   // per-node metrics (copied from SEMQuad at creation); used when `nodal` is set
   double *ndsdx = nullptr, *nJ = nullptr, *nnHat = nullptr, *nnScale = nullptr;
   bool affine = true, nodal = false;
+  bool anyBoundary = false;  // some element side is a physical boundary (neighbour code < 0)
   int32_t *nbr = nullptr;
   int cur = 0;  // trace buffer holding the traces of the current S
   Params p{};
@@ -1036,6 +1065,7 @@ struct selfamd_model2d {
     // element order alternates with the trace buffer parity, i.e. from stage to stage
     a.reverse = tile_alternates(kind, N, a.nEl) ? traceIn : 0;
     a.perElemDt = 0;
+    a.anyBoundary = anyBoundary;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(N, mode, nodal, a, stream);
@@ -1211,6 +1241,7 @@ int selfamd_model2d_create(selfamd_model2d **out, int kind, int N, int nEl, cons
   Model *m = new Model();
   m->kind = kind; m->N = N; m->nEl = nEl;
   m->hnbr = nbr;
+  m->anyBoundary = std::any_of(nbr.begin(), nbr.end(), [](int32_t c) { return c < 0; });
   m->affine = affine;
   m->nodal = !affine;
   m->nVar = selfamd_model2d_nvar(kind);
@@ -1774,6 +1805,7 @@ struct selfamd_batch {
   std::vector<double> dt;
   std::vector<BatchGroup> groups;
   Params p{};
+  bool anyBoundary = false;  // some run's mesh has a physical boundary side
   hipStream_t stream = nullptr;
   int cur = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1788,6 +1820,7 @@ struct selfamd_batch {
     a.rkA = kRk3A[stage]; a.rkG = kRk3G[stage]; a.rkGdt = 0.0; a.perElemDt = 1;
     a.writeG = stage != 2;
     a.reverse = tile_alternates(kind, g.N, g.nEl) ? tin : 0;
+    a.anyBoundary = anyBoundary;
     a.p = p;
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);
@@ -1835,6 +1868,7 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
   selfamd_batch *b = new selfamd_batch();
   b->kind = m0->kind;
   b->p = m0->p;
+  for (int r = 0; r < nModels; r++) b->anyBoundary = b->anyBoundary || models[r]->anyBoundary;
   b->models.assign(models, models + nModels);
   b->dt.assign(dt, dt + nModels);
   auto fail = [&](const char *what, int code = kHipError) {
