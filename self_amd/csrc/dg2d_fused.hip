@@ -345,15 +345,15 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // strides of 7 were slower, 72.3-72.7 us).
   constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NP == 6 ? NN + 2 : NN;
   auto slot = [](int i_, int j_) { return SWZ ? NP * j_ + (i_ ^ (j_ & 6)) : i_ + NP * j_; };
-  __shared__ double sQ[NT][EPB][NNS];
-  __shared__ double sF1[NV][EPB][NNS];
-  __shared__ double sF2[NV][EPB][NNS];
-  __shared__ double sAux[USE_AUX ? EPB : 1][USE_AUX ? NNS : 1];
+  __shared__ __align__(16) double sQ[NT][EPB][NNS];
+  __shared__ __align__(16) double sF1[NV][EPB][NNS];
+  __shared__ __align__(16) double sF2[NV][EPB][NNS];
+  __shared__ __align__(16) double sAux[USE_AUX ? EPB : 1][USE_AUX ? NNS : 1];
   // nodal mode: static field times the node's contravariant metric, for GradientSF
   constexpr bool GRADN = NODAL && Phys::kAuxGrad;
   __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NNS : 1];
   __shared__ double sBF[NV][EPB][NF];
-  __shared__ double sM[2 * NP * MRS + 4 * NP];  // dgM, D (rows of MRS), bM(2P), qW, 1/qW
+  __shared__ __align__(16) double sM[2 * NP * MRS + 4 * NP];  // dgM, D (rows of MRS), bM(2P), qW, 1/qW
 
   const int tid = threadIdx.x;
   const int el = tid / NN, node = tid % NN;
@@ -366,11 +366,15 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   }
   if (a.reverse) blk = (int)gridDim.x - 1 - blk;
   const int pe = blk * EPB + el;  // position in the launch's element list
-  const bool nodeActive = el < EPB && pe < a.nEl;
+  bool nodeActive = el < EPB && pe < a.nEl;
   int e = a.elems ? (nodeActive ? a.elems[pe] : 0) : pe;
-  // one element per wavefront: its index (and every address derived from it) is uniform, so
-  // the element record and base addresses can live in scalar registers
-  if constexpr (NN == 64) e = __builtin_amdgcn_readfirstlane(e);
+  // one element per wavefront: its index, whether it is active, and every address and record
+  // value derived from it are uniform, so they live in scalar registers and the branches on
+  // them are scalar
+  if constexpr (NN == 64) {
+    e = __builtin_amdgcn_readfirstlane(e);
+    nodeActive = __builtin_amdgcn_readfirstlane((int)nodeActive) != 0;
+  }
   const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
 
   // ---- loads: node lanes ----
@@ -603,21 +607,71 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
 
   // ---- phase 2: divergence, source, update ----
   if (nodeActive) {
-    double gx = 0.0, gy = 0.0;
-    if (Phys::kAuxGrad && a.aux) {
-      // GradientSF of the static field (src/SELF_MappedData.f90:1080-1149)
-      if constexpr (GRADN) {
-        for (int ii = 0; ii < NP; ii++) {
-          gx = gx + Dm[ii + MRS * i] * sGb[0][el][slot(ii, j)] + Dm[ii + MRS * j] * sGb[2][el][slot(i, ii)];
-          gy = gy + Dm[ii + MRS * i] * sGb[1][el][slot(ii, j)] + Dm[ii + MRS * j] * sGb[3][el][slot(i, ii)];
+    // One pass over the line index ii for every sum of the phase: the volume terms of the weak
+    // divergence of each variable (VectorDGDivergence_2D, src/SELF_Lagrange.f90:1589-1625; xi
+    // and eta sums in separate FMA chains) and GradientSF of the static field
+    // (src/SELF_MappedData.f90:1080-1149).  Each matrix entry is read from LDS once for all
+    // variables; every sum keeps its own accumulation order.
+    constexpr bool GRAD = Phys::kAuxGrad;
+    const bool grad = GRAD && a.aux;
+    double dxs[NV], des[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) { dxs[v] = 0.0; des[v] = 0.0; }
+    double gx = 0.0, gy = 0.0, dxi = 0.0, deta = 0.0;
+    // one step of the sums at line index ii: matrix entries and row-wise (xi) operands come in
+    // as pairs (ii, ii + 1) when NP is even (16-byte LDS reads: the pairs are adjacent and
+    // aligned in every node and matrix layout); the loop is not unrolled further, so the
+    // compiler does not hoist every LDS read of the phase (that took 192 VGPRs)
+    auto line = [&](int ii, double ai, double aj, const double *f1, double di, double dj, double bx) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        dxs[v] = dxs[v] + ai * f1[v];
+        des[v] = des[v] + aj * sF2[v][el][slot(i, ii)];
+      }
+      if (GRAD && grad) {
+        if constexpr (GRADN) {
+          gx = gx + di * sGb[0][el][slot(ii, j)] + dj * sGb[2][el][slot(i, ii)];
+          gy = gy + di * sGb[1][el][slot(ii, j)] + dj * sGb[3][el][slot(i, ii)];
+        } else {
+          dxi = dxi + di * bx;
+          deta = deta + dj * sAux[el][slot(i, ii)];
         }
-      } else {
+      }
+    };
+    if constexpr (NP % 2 == 0) {
+#pragma unroll 1
+      for (int ii = 0; ii < NP; ii += 2) {
+        const double2 ai = *reinterpret_cast<const double2 *>(&dgM[ii + MRS * i]);
+        const double2 aj = *reinterpret_cast<const double2 *>(&dgM[ii + MRS * j]);
+        double2 di = {0.0, 0.0}, dj = {0.0, 0.0}, bx = {0.0, 0.0};
+        if (GRAD && grad) {
+          di = *reinterpret_cast<const double2 *>(&Dm[ii + MRS * i]);
+          dj = *reinterpret_cast<const double2 *>(&Dm[ii + MRS * j]);
+          if constexpr (!GRADN) bx = *reinterpret_cast<const double2 *>(&sAux[el][slot(ii, j)]);
+        }
+        double f1a[NV], f1b[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const double2 f = *reinterpret_cast<const double2 *>(&sF1[v][el][slot(ii, j)]);
+          f1a[v] = f.x;
+          f1b[v] = f.y;
+        }
+        line(ii, ai.x, aj.x, f1a, di.x, dj.x, bx.x);
+        line(ii + 1, ai.y, aj.y, f1b, di.y, dj.y, bx.y);
+      }
+    } else {
+#pragma unroll 2
+      for (int ii = 0; ii < NP; ii++) {
+        double f1[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) f1[v] = sF1[v][el][slot(ii, j)];
+        line(ii, dgM[ii + MRS * i], dgM[ii + MRS * j], f1, Dm[ii + MRS * i], Dm[ii + MRS * j],
+             USE_AUX ? sAux[el][slot(ii, j)] : 0.0);
+      }
+    }
+    if (GRAD && grad) {
+      if constexpr (!GRADN) {
         // affine element: the metric terms are constant, so they factor out of the sums
-        double dxi = 0.0, deta = 0.0;
-        for (int ii = 0; ii < NP; ii++) {
-          dxi = dxi + Dm[ii + MRS * i] * sAux[el][slot(ii, j)];
-          deta = deta + Dm[ii + MRS * j] * sAux[el][slot(i, ii)];
-        }
         gx = dsdx[0] * dxi + dsdx[2] * deta;
         gy = dsdx[1] * dxi + dsdx[3] * deta;
       }
@@ -632,15 +686,8 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
     const double bi1 = bM1[i], bi0 = bM0[i], bj1 = bM1[j], bj0 = bM0[j];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-      // VectorDGDivergence_2D (src/SELF_Lagrange.f90:1589-1625) then JacobianWeight
-      // xi and eta sums accumulate separately (two independent FMA chains)
-      double dxs = 0.0, des = 0.0;
-#pragma unroll
-      for (int ii = 0; ii < NP; ii++) {
-        dxs = dxs + dgM[ii + MRS * i] * sF1[v][el][slot(ii, j)];
-        des = des + dgM[ii + MRS * j] * sF2[v][el][slot(i, ii)];
-      }
-      double d = dxs + des;
+      // surface terms of the weak divergence, then JacobianWeight
+      double d = dxs[v] + des[v];
       d = d + (bi1 * sBF[v][el][NP + j] + bi0 * sBF[v][el][3 * NP + j]) * iwi +
           (bj1 * sBF[v][el][2 * NP + i] + bj0 * sBF[v][el][i]) * iwj;
       d = d * invJ;
