@@ -389,6 +389,14 @@ int selfamd_mlp_loss_grad(selfamd_mlp *mlp, const float *X_dev, const float *T_d
                           void *stream);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected; step counts from 1). */
 int selfamd_mlp_adam_step(selfamd_mlp *mlp, float lr, int step, void *stream);
+/* Backward pass from a given output cotangent dY[batch][nOut] (device): forward of X, then the
+ * parameter gradients written to the gradient buffer (accumulate = 0) or added to it
+ * (accumulate = 1), and the input cotangent dX[batch][nIn] (device, may be NULL).  For
+ * training through a solver (the closure's a-posteriori training).  No reference counterpart. */
+int selfamd_mlp_backward(selfamd_mlp *mlp, const float *X_dev, const float *dY_dev, int batch, int accumulate,
+                         float *dX_dev, void *stream);
+/* Sets the gradient buffer to zero (before accumulating backward passes). */
+int selfamd_mlp_zero_grads(selfamd_mlp *mlp, void *stream);
 
 /* L2 projection of interior(0:Nf,0:Nf,nVar,(r nxc)(r nyc)) on a structured raster-ordered
  * mesh onto interior(0:Nc,0:Nc,nVar,nxc nyc): each coarse element covers r x r fine elements.
@@ -404,6 +412,26 @@ int selfamd_project_l2_structured_2d(const double *fine_dev, double *coarse_dev,
  * arrays are host memory; mlp = NULL detaches.  The model keeps a pointer to the network. */
 int selfamd_model2d_set_closure(selfamd_model2d *model, selfamd_mlp *mlp, const float *inMean, const float *inScale,
                                 const float *outMean, const float *outScale);
+/* Closure training through the solver (no reference counterpart).
+ * selfamd_model2d_stage: one low-storage RK3 stage (0, 1, 2 in turn; a step is three calls)
+ * of ForwardStep's arithmetic, closure included; before it the stage's input state is copied to
+ * Ssave_dev (interior layout) and the lagged SW face normal velocities it reads to unSave_dev
+ * ((0:N,4,nEl), affine mode), either may be NULL.  Synchronises the model's stream. */
+int selfamd_model2d_stage(selfamd_model2d *model, int stage, double dt, double *Ssave_dev, double *unSave_dev);
+/* mu = (d dSdt / dS)^T lambda: the adjoint of the nonlinear shallow-water tendency
+ * (CalculateTendency_Model2D, src/SELF_Model2D.f90:816-859, with the plug-ins of
+ * src/models/SELF_ShallowWater.f90) at state S with the lagged normal velocities un (as saved
+ * by selfamd_model2d_stage); affine meshes whose sides are all interior (periodic).  With
+ * Sprev (the previous stage's input state, from which un was formed) also the lag path,
+ * muPrev = (d dSdt / d Sprev)^T lambda; both NULL holds un fixed.  The closure's correction is
+ * not included (selfamd_model2d_closure_vjp).  Synchronises the model's stream. */
+int selfamd_model2d_tendency_vjp(selfamd_model2d *model, const double *S_dev, const double *un_dev,
+                                 const double *lambda_dev, double *mu_dev, const double *Sprev_dev, double *muPrev_dev);
+/* Adjoint of the attached closure's correction at state S for its cotangent lambda: the
+ * network's parameter gradients are ADDED to its gradient buffer (selfamd_mlp_zero_grads
+ * clears it) and the state cotangent through the network's inputs is ADDED to mu.
+ * Synchronises the model's stream. */
+int selfamd_model2d_closure_vjp(selfamd_model2d *model, const double *S_dev, const double *lambda_dev, double *mu_dev);
 /* The closure's normalised features of the current state: X(nEl, n_in) FP32 device. */
 int selfamd_model2d_closure_features(selfamd_model2d *model, float *X_dev);
 

@@ -348,6 +348,35 @@ class Model2D:
         call("selfamd_model2d_integral", self._h, int(var), ctypes.byref(v))
         return v.value
 
+    def Stage(self, stage, dt, Ssave=None, unSave=None):
+        """One low-storage RK3 stage (0, 1, 2 in turn), closure included; the stage's input
+        state and lagged face normal velocities are copied to the given DeviceArrays first."""
+        call("selfamd_model2d_stage", self._h, int(stage), float(dt), Ssave.ptr if Ssave is not None else None,
+             unSave.ptr if unSave is not None else None)
+
+    @property
+    def lag_shape(self):
+        """Shape of the lagged SW face normal velocities of the affine engine, (nElem, 4, N+1)."""
+        return (self.nElem, 4, self.N + 1)
+
+    def TendencyVJP(self, S, un, lam, mu=None, Sprev=None, muPrev=None):
+        """mu = (d dSdt / dS)^T lam at state S with the lagged normal velocities un (SW, affine,
+        periodic); with Sprev (the previous stage's input, from which un was formed) also the
+        lag path into the DeviceArray muPrev.  Arrays host or DeviceArrays; returns mu."""
+        Sd, ud, ld = _dev(S, self.shape), _dev(un, self.lag_shape), _dev(lam, self.shape)
+        mu = mu if mu is not None else DeviceArray(self.shape)
+        Spd = _dev(Sprev, self.shape) if Sprev is not None else None
+        call("selfamd_model2d_tendency_vjp", self._h, _ptr(Sd), _ptr(ud), _ptr(ld), _ptr(mu),
+             _ptr(Spd) if Spd is not None else None, _ptr(muPrev) if muPrev is not None else None)
+        return mu
+
+    def ClosureVJP(self, S, lam, mu):
+        """Adjoint of the attached closure's correction at state S: adds the network's parameter
+        gradients to its gradient buffer and the state cotangent to the DeviceArray mu."""
+        Sd, ld = _dev(S, self.shape), _dev(lam, self.shape)
+        call("selfamd_model2d_closure_vjp", self._h, _ptr(Sd), _ptr(ld), _ptr(mu))
+        return mu
+
     def IntegralNoAlloc(self, var, out):
         """Integral(var) written to the device double `out` (a DeviceArray) on the model's
         stream by the library's no-allocation entry point; does not synchronise."""
@@ -673,6 +702,19 @@ class MLP:
 
     def adam_step(self, lr, step):
         call("selfamd_mlp_adam_step", self._h, float(lr), int(step), None)
+
+    def backward(self, X, dY, accumulate=False, want_dX=True):
+        """Parameter gradients (written, or added with accumulate) and the input cotangent
+        (host array, or None) of the output cotangent dY for inputs X (host or DeviceArray)."""
+        Xd = X if isinstance(X, DeviceArray) else DeviceArray.from_host(np.asarray(X, np.float32))
+        Yd = dY if isinstance(dY, DeviceArray) else DeviceArray.from_host(np.asarray(dY, np.float32))
+        rows = Xd.shape[0]
+        dX = DeviceArray((rows, self.nIn), np.float32) if want_dX else None
+        call("selfamd_mlp_backward", self._h, Xd.ptr, Yd.ptr, rows, int(bool(accumulate)), dX.ptr if dX else None, None)
+        return dX.to_host() if dX else None
+
+    def zero_grads(self):
+        call("selfamd_mlp_zero_grads", self._h, None)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -85,6 +85,9 @@ SIGNATURES = {
     "selfamd_model2d_synchronize": [c_void_p],
     "selfamd_model2d_time_steps": [c_void_p, c_int, c_double, c_void_p],
     "selfamd_model2d_stream": [c_void_p, c_void_p],
+    "selfamd_model2d_stage": [c_void_p, c_int, c_double, c_void_p, c_void_p],
+    "selfamd_model2d_tendency_vjp": [c_void_p] * 7,
+    "selfamd_model2d_closure_vjp": [c_void_p] * 4,
     "selfamd_model2d_integral_noalloc": [c_void_p, c_int, c_void_p],
     "selfamd_model2d_get_time": [c_void_p, c_void_p],
     "selfamd_model2d_set_time": [c_void_p, c_double],
@@ -103,6 +106,8 @@ SIGNATURES = {
     "selfamd_mlp_forward": [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
     "selfamd_mlp_loss_grad": [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
     "selfamd_mlp_adam_step": [c_void_p, ctypes.c_float, c_int, c_void_p],
+    "selfamd_mlp_backward": [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "selfamd_mlp_zero_grads": [c_void_p, c_void_p],
     "selfamd_project_l2_structured_2d": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p],
     "selfamd_model2d_set_closure": [c_void_p] * 6,
     "selfamd_model2d_closure_features": [c_void_p, c_void_p],

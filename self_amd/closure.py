@@ -193,6 +193,117 @@ def train(mlp, Xn, Tn, epochs=20, batch=4096, lr=1e-3, lr_final=1e-4, seed=0, lo
     return hist
 
 
+# Williamson low-storage RK3 coefficients (src/SELF_Model.f90:25-27)
+RK3_A = (0.0, -5.0 / 9.0, -153.0 / 128.0)
+RK3_G = (1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0)
+
+
+def l2_weights(cf):
+    """Quadrature weights of the coarse L2 norm per node, shape (nEl, 1, Nc+1, Nc+1)."""
+    w2 = np.outer(cf.qW_c, cf.qW_c)
+    return (cf.J_c[:, None, None] * w2[None])[:, None]
+
+
+def rollout_loss_grad(C, mlp, P0, truths, dt, weights, want_grad=True):
+    """A-posteriori loss of K = len(truths) closure-corrected coarse RK3 steps from P0,
+    L = (1/K) sum_k sum_v ||S_k - T_k||^2 (quadrature-weighted), and with want_grad its
+    gradient with respect to the network parameters, left in the network's gradient buffer.
+
+    The gradient is the adjoint of the K steps on the GPU: per stage (last to first) the
+    transposed RK3 update, the tendency adjoint (selfamd_model2d_tendency_vjp, the lagged
+    normal velocities held at their forward values) and the closure adjoint
+    (selfamd_model2d_closure_vjp: parameter gradients and the state cotangent through the
+    network inputs).  Low-storage RK3 stage m: G_m = a_m G_(m-1) + f(S_(m-1)),
+    S_m = S_(m-1) + g_m dt G_m; backwards, with lG~ = lG_m + g_m dt lS_m:
+    lS_(m-1) = lS_m + f'(S_(m-1))^T lG~ and lG_(m-1) = a_m lG~."""
+    K = len(truths)
+    shape = C.shape
+    C.SetSolution(P0)
+    Ss = [[DeviceArray(shape) for _ in range(3)] for _ in range(K)] if want_grad else None
+    Us = [[DeviceArray(C.lag_shape) for _ in range(3)] for _ in range(K)] if want_grad else None
+    loss, deltas = 0.0, []
+    for k in range(K):
+        for st in range(3):
+            C.Stage(st, dt, Ss[k][st] if want_grad else None, Us[k][st] if want_grad else None)
+        r = C.solution - truths[k]
+        loss += float(np.sum(weights * r * r)) / K
+        deltas.append(2.0 * weights * r / K)
+    if not want_grad:
+        return loss
+    mlp.zero_grads()
+    lS, lG = deltas[-1].copy(), np.zeros(shape)
+    mu, muP = DeviceArray(shape), DeviceArray(shape)
+    lGt_d = DeviceArray(shape)
+    pending = 0.0  # lag-path cotangent of the previous stage's input, from the stage after it
+    for k in reversed(range(K)):
+        for st in (2, 1, 0):
+            lGt = lG + RK3_G[st] * dt * lS
+            lGt_d.copy_from_host(lGt)
+            # the lagged velocities of stage (k, st) were formed from the previous stage's input
+            # (none before the first stage: SetSolution forms them from P0, independent of the
+            # network)
+            prev = Ss[k][st - 1] if st > 0 else (Ss[k - 1][2] if k > 0 else None)
+            C.TendencyVJP(Ss[k][st], Us[k][st], lGt_d, mu, Sprev=prev, muPrev=muP if prev is not None else None)
+            C.ClosureVJP(Ss[k][st], lGt_d, mu)
+            lS = lS + mu.to_host() + pending
+            pending = muP.to_host() if prev is not None else 0.0
+            lG = RK3_A[st] * lGt
+        if k > 0:
+            lS += deltas[k - 1]
+    return loss
+
+
+def rollout_windows(cf, seeds, K, every, n_steps=500, log=None):
+    """Training windows from fine runs: the projected fine state at every `every`-th coarse step
+    and the projected states of the K coarse steps after it.  Returns a list of
+    (topography of the run, P0, [T_1..T_K])."""
+    out = []
+    for seed in seeds:
+        F, topo = cf.fine_model(seed)
+        states = [cf.project(F.solution_device()).to_host()]
+        for step in range(1, n_steps + 1):
+            F.ForwardStep(cf.k, cf.dt_f)
+            if step % every <= K:
+                states.append(cf.project(F.solution_device()).to_host())
+            else:
+                states.append(None)
+        for t0 in range(0, n_steps - K + 1, every):
+            win = states[t0:t0 + K + 1]
+            if all(w is not None and np.isfinite(w).all() for w in win):
+                out.append((topo, win[0], win[1:]))
+        if log:
+            log(f"closure windows: seed {seed} done ({len(out)} windows)")
+        del F
+    return out
+
+
+def train_aposteriori(cf, mlp, norm, windows, iters, lr=1e-4, lr_final=None, seed=0, log=None, log_every=50):
+    """Adam on the a-posteriori rollout loss over randomly drawn windows (one window per
+    step); the network corrects the coarse run through norm.arrays() at alpha = 1.  Returns
+    the per-step losses."""
+    rng = np.random.default_rng(seed)
+    weights = l2_weights(cf)
+    K = len(windows[0][2])
+    lr_final = lr if lr_final is None else lr_final
+    models = {}
+    hist = []
+    for it in range(1, iters + 1):
+        topo, P0, truths = windows[int(rng.integers(len(windows)))]
+        key = id(topo)
+        if key not in models:
+            C = cf.coarse_model(topo)
+            C.SetClosure(mlp, *norm.arrays(1.0))
+            models[key] = C
+        C = models[key]
+        loss = rollout_loss_grad(C, mlp, P0, truths, cf.dt_c, weights)
+        rate = lr * (lr_final / lr) ** ((it - 1) / max(1, iters - 1))
+        mlp.adam_step(rate, it)
+        hist.append(loss)
+        if log and it % log_every == 0:
+            log(f"closure a-posteriori: step {it}/{iters} loss {np.mean(hist[-log_every:]):.4e}")
+    return hist
+
+
 def skill(mlp, Xn, Tn, rows=65536):
     """A priori skill on normalised data: 1 - sum (y - t)^2 / sum (t - mean t)^2 (R^2 over all
     outputs) on up to `rows` samples."""

@@ -88,12 +88,13 @@ __global__ void __launch_bounds__(256) k_gemm(const float *__restrict__ A, const
   }
 }
 
-// out[i] = sum_s parts[s*n + i] (fixed order)
-__global__ void k_sum_splits(const float *__restrict__ parts, float *__restrict__ out, int n, int splits) {
+// out[i] = sum_s parts[s*n + i] (fixed order), added to out[i] when accumulate is set
+__global__ void k_sum_splits(const float *__restrict__ parts, float *__restrict__ out, int n, int splits,
+                             int accumulate) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float s = 0.f;
     for (int k = 0; k < splits; k++) s += parts[(size_t)k * n + i];
-    out[i] = s;
+    out[i] = accumulate ? out[i] + s : s;
   }
 }
 
@@ -180,16 +181,43 @@ hipError_t forward_impl(selfamd_mlp *m, const float *X, float *Y, int B, hipStre
   return gemm<false, false, 1>(m->H2, m->W3(), Y, nullptr, m->b3(), B, m->nOut, m->nH, m->nH, m->nOut, 1, st);
 }
 
-// weight gradient gW[in, out] = A^T D over B rows, split over blocks then summed in order
+// weight gradient gW[in, out] (+)= A^T D over B rows, split over blocks then summed in order
 hipError_t weight_grad(selfamd_mlp *m, const float *A, const float *D, float *gW, int B, int nin, int nout,
-                       hipStream_t st) {
+                       hipStream_t st, int accumulate = 0) {
   int splits = (B + 255) / 256;
   if (splits > m->splitsMax) splits = m->splitsMax;
   if (splits < 1) splits = 1;
   hipError_t e = gemm<true, false, 0>(A, D, m->parts, nullptr, nullptr, nin, nout, B, nin, nout, splits, st);
   if (e) return e;
   const int n = nin * nout;
-  hipLaunchKernelGGL(k_sum_splits, (n + 255) / 256, 256, 0, st, m->parts, gW, n, splits);
+  hipLaunchKernelGGL(k_sum_splits, (n + 255) / 256, 256, 0, st, m->parts, gW, n, splits, accumulate);
+  return hipGetLastError();
+}
+
+// Backward pass from the output cotangent dY (overwritten) through the activations that the
+// last forward_impl of the same rows left in Z1, H1, Z2, H2: parameter gradients written to (or,
+// with accumulate, added to) m->grads, and the input cotangent dX = D1 W1^T when dX is given.
+hipError_t backward_impl(selfamd_mlp *m, const float *X, float *dY, int B, int accumulate, float *dX,
+                         hipStream_t st) {
+  const int H = m->nH, nO = m->nOut;
+  float *g = m->grads;
+  hipError_t e;
+  // layer 3
+  if ((e = weight_grad(m, m->H2, dY, g + m->off_W3(), B, H, nO, st, accumulate))) return e;
+  if ((e = weight_grad(m, m->ones, dY, g + m->off_b3(), B, 1, nO, st, accumulate))) return e;
+  if ((e = gemm<false, true, 0>(dY, m->W3(), m->D2, nullptr, nullptr, B, H, nO, nO, nO, 1, st))) return e;
+  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D2, m->Z2,
+                     (size_t)B * H);
+  // layer 2
+  if ((e = weight_grad(m, m->H1, m->D2, g + m->off_W2(), B, H, H, st, accumulate))) return e;
+  if ((e = weight_grad(m, m->ones, m->D2, g + m->off_b2(), B, 1, H, st, accumulate))) return e;
+  if ((e = gemm<false, true, 0>(m->D2, m->W2(), m->D1, nullptr, nullptr, B, H, H, H, H, 1, st))) return e;
+  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D1, m->Z1,
+                     (size_t)B * H);
+  // layer 1
+  if ((e = weight_grad(m, X, m->D1, g, B, m->nIn, H, st, accumulate))) return e;
+  if ((e = weight_grad(m, m->ones, m->D1, g + m->off_b1(), B, 1, H, st, accumulate))) return e;
+  if (dX && (e = gemm<false, true, 0>(m->D1, m->W1(), dX, nullptr, nullptr, B, m->nIn, H, H, H, 1, st))) return e;
   return hipGetLastError();
 }
 
@@ -283,23 +311,8 @@ int selfamd_mlp_loss_grad(selfamd_mlp *m, const float *X, const float *T, int B,
   int blocks = (int)std::min<size_t>((n + 255) / 256, 1024);
   hipLaunchKernelGGL(k_mse, blocks, 256, 0, st, m->Y, T, m->dY, m->lossParts, n);
   SELFAMD_HIP(hipGetLastError());
-  float *g = m->grads;
-  // layer 3
-  SELFAMD_HIP(weight_grad(m, m->H2, m->dY, g + m->off_W3(), B, H, nO, st));
-  SELFAMD_HIP(weight_grad(m, m->ones, m->dY, g + m->off_b3(), B, 1, nO, st));
-  SELFAMD_HIP((gemm<false, true, 0>(m->dY, m->W3(), m->D2, nullptr, nullptr, B, H, nO, nO, nO, 1, st)));
-  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D2, m->Z2,
-                     (size_t)B * H);
-  // layer 2
-  SELFAMD_HIP(weight_grad(m, m->H1, m->D2, g + m->off_W2(), B, H, H, st));
-  SELFAMD_HIP(weight_grad(m, m->ones, m->D2, g + m->off_b2(), B, 1, H, st));
-  SELFAMD_HIP((gemm<false, true, 0>(m->D2, m->W2(), m->D1, nullptr, nullptr, B, H, H, H, H, 1, st)));
-  hipLaunchKernelGGL(k_gelu_back, (int)std::min<size_t>(((size_t)B * H + 255) / 256, 4096), 256, 0, st, m->D1, m->Z1,
-                     (size_t)B * H);
-  // layer 1
-  SELFAMD_HIP(weight_grad(m, X, m->D1, g, B, m->nIn, H, st));
-  SELFAMD_HIP(weight_grad(m, m->ones, m->D1, g + m->off_b1(), B, 1, H, st));
-  SELFAMD_HIP(hipGetLastError());
+  (void)H;
+  SELFAMD_HIP(backward_impl(m, X, m->dY, B, 0, nullptr, st));
   if (loss) {
     std::vector<float> parts(blocks);
     SELFAMD_HIP(hipMemcpyAsync(parts.data(), m->lossParts, 4 * blocks, hipMemcpyDeviceToHost, st));
@@ -308,6 +321,25 @@ int selfamd_mlp_loss_grad(selfamd_mlp *m, const float *X, const float *T, int B,
     for (float p : parts) s += p;
     *loss = (float)(s / (double)n);
   }
+  return kOk;
+}
+
+int selfamd_mlp_backward(selfamd_mlp *m, const float *X, const float *dY, int B, int accumulate, float *dX,
+                         void *stream) {
+  SELFAMD_REQUIRE(m && X && dY && B >= 1 && B <= m->maxB, "mlp_backward: bad arguments (B=%d, max %d)", B,
+                  m ? m->maxB : 0);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SELFAMD_HIP(forward_impl(m, X, m->Y, B, st));
+  SELFAMD_HIP(hipMemcpyAsync(m->dY, dY, sizeof(float) * (size_t)B * m->nOut, hipMemcpyDeviceToDevice, st));
+  SELFAMD_HIP(backward_impl(m, X, m->dY, B, accumulate, dX, st));
+  return kOk;
+}
+
+int selfamd_mlp_zero_grads(selfamd_mlp *m, void *stream) {
+  SELFAMD_REQUIRE(m, "mlp_zero_grads: null argument");
+  hipLaunchKernelGGL(k_fill, (int)std::min<size_t>((m->nParams + 255) / 256, 4096), 256, 0,
+                     reinterpret_cast<hipStream_t>(stream), m->grads, 0.0f, m->nParams);
+  SELFAMD_HIP(hipGetLastError());
   return kOk;
 }
 
@@ -578,6 +610,97 @@ hipError_t closure_evaluate(const ClosureBuffers &c, const double *S, const doub
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_closure_shift, 1, 256, 0, st, c.elemMass, c.shift, c.invArea, nEl);
+  return hipGetLastError();
+}
+
+namespace {
+constexpr int kVjpBlock = 256;
+
+// per-block sums of lam over the mass variable's nodes (fixed order), then the total
+__global__ void __launch_bounds__(kVjpBlock) k_vjp_mass_partial(const double *__restrict__ lam, double *__restrict__ part,
+                                                               int NN, int nVar, int massVar, size_t nNodes) {
+  __shared__ double red[kVjpBlock];
+  const size_t n = blockIdx.x * (size_t)kVjpBlock + threadIdx.x;
+  double v = 0.0;
+  if (n < nNodes) {
+    const size_t e = n / NN;
+    v = lam[(e * nVar + massVar) * NN + n % NN];
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kVjpBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kVjpBlock) k_vjp_sum(double *__restrict__ part, int n) {
+  __shared__ double red[kVjpBlock];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += kVjpBlock) v += part[k];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kVjpBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[n] = red[0];
+}
+
+// Output cotangent.  Forward: corr_k = y_k sdOut_k + muOut_k, and for the mass variable
+// corr' = corr - invArea sum_n omega_n corr_n (omega = w_i w_j J); so
+// dY_k = sdOut_k (lam_k - omega_k invArea sum_n lam_n) on the mass variable, sdOut_k lam_k elsewhere.
+__global__ void k_vjp_dY(const double *__restrict__ lam, const double *__restrict__ total, const float *__restrict__ sdOut,
+                         const double *__restrict__ geom, const double *__restrict__ qW, float *__restrict__ dY, int N,
+                         int nVar, int massVar, double invArea, size_t n) {
+  const int P = N + 1, NN = P * P, nOut = nVar * NN;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(t % nOut);
+    const size_t e = t / nOut;
+    double l = lam[t];
+    if (k / NN == massVar) {
+      const int node = k % NN;
+      l -= qW[node % P] * qW[node / P] * geom[e * kGeomStride + 4] * invArea * total[0];
+    }
+    dY[t] = (float)(l * (double)sdOut[k]);
+  }
+}
+
+// mu += dX / inScale over the nodal-state inputs (the metric inputs do not depend on the state)
+__global__ void k_vjp_mu(const float *__restrict__ dX, const float *__restrict__ sdIn, double *__restrict__ mu, int nIn,
+                         int nOut, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(t % nOut);
+    const size_t e = t / nOut;
+    mu[t] += (double)dX[e * nIn + k] / (double)sdIn[k];
+  }
+}
+}  // namespace
+
+size_t closure_vjp_partials(int N, int nEl) {
+  const size_t nNodes = (size_t)(N + 1) * (N + 1) * nEl;
+  return (nNodes + kVjpBlock - 1) / kVjpBlock + 1;
+}
+
+hipError_t closure_vjp(const ClosureBuffers &c, const double *S, const double *geom, const double *qW,
+                       const double *lam, double *mu, int N, int nVar, int massVar, int nEl, hipStream_t st) {
+  hipError_t e;
+  const int NN = (N + 1) * (N + 1);
+  if ((e = closure_features(S, geom, c.inMean, c.inScale, c.X, N, nVar, nEl, st)) != hipSuccess) return e;
+  const size_t nNodes = (size_t)NN * nEl, nb = (nNodes + kVjpBlock - 1) / kVjpBlock;
+  hipLaunchKernelGGL(k_vjp_mass_partial, (unsigned)nb, kVjpBlock, 0, st, lam, c.vjpPartial, NN, nVar, massVar, nNodes);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_vjp_sum, 1, kVjpBlock, 0, st, c.vjpPartial, (int)nb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const size_t n = (size_t)c.nOut * nEl;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65535);
+  hipLaunchKernelGGL(k_vjp_dY, blocks, 256, 0, st, lam, c.vjpPartial + nb, c.outScale, geom, qW, c.dY, N, nVar, massVar,
+                     c.invArea, n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = forward_impl(c.mlp, c.X, c.mlp->Y, nEl, st)) != hipSuccess) return e;
+  if ((e = backward_impl(c.mlp, c.X, c.dY, nEl, 1, c.dX, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_vjp_mu, blocks, 256, 0, st, c.dX, c.inScale, mu, c.nIn, c.nOut, n);
   return hipGetLastError();
 }
 

@@ -17,6 +17,9 @@ struct ClosureBuffers {
   double *elemMass = nullptr;                // per-element integral of the mass-variable correction
   double *shift = nullptr;                   // one double: global mean of the mass correction
   double invArea = 0.0;                      // 1 / (sum over elements of the element area in xi-space times J)
+  // adjoint (closure_vjp) work space
+  float *dY = nullptr, *dX = nullptr;        // output / input cotangents, one row per element
+  double *vjpPartial = nullptr;              // block partial sums of the mass-variable cotangent, then the total
 };
 
 // Normalised MLP inputs of state S: per element the nVar*(N+1)^2 nodal values, then the
@@ -27,6 +30,14 @@ hipError_t closure_features(const double *S, const double *geom, const float *in
 // features -> MLP -> correction (+ per-element mass integrals and the global mean shift)
 hipError_t closure_evaluate(const ClosureBuffers &c, const double *S, const double *geom, const double *qW, int N,
                             int nVar, int massVar, int nEl, hipStream_t st);
+
+// Adjoint of the correction that closure_evaluate adds to dS/dt at state S: for the cotangent
+// lam of that correction (interior layout), the network's parameter gradients are added to its
+// gradient buffer and the state cotangent through the network inputs is added to mu.
+hipError_t closure_vjp(const ClosureBuffers &c, const double *S, const double *geom, const double *qW,
+                       const double *lam, double *mu, int N, int nVar, int massVar, int nEl, hipStream_t st);
+// number of block partial sums closure_vjp needs (+1 for the total)
+size_t closure_vjp_partials(int N, int nEl);
 
 hipError_t mlp_forward(selfamd_mlp *m, const float *X, float *Y, int B, hipStream_t st);
 int mlp_shape(const selfamd_mlp *m, int *nIn, int *nOut, int *maxB);

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "closure.hpp"
+#include "sw_adjoint.hpp"
 #include "../../include/self_amd.h"
 
 using namespace selfamd;
@@ -1070,6 +1071,7 @@ struct selfamd_model2d {
   ClosureBuffers *closure = nullptr;
   double *intScratch = nullptr;  // partial sums of the state integrals (allocated on first use)
   size_t intScratchLen = 0;
+  double *adjTrace = nullptr, *adjChi = nullptr;  // tendency adjoint: traces of its state, neighbour-trace cotangents
 
   // Banded stepping: the elements split into bands of whole BFS levels of the element graph
   // (neighbours differ by at most one band); one RK3 step runs stage s of band b in round
@@ -1194,7 +1196,8 @@ using Model = selfamd_model2d;
 namespace {
 void free_closure(ClosureBuffers *c) {
   if (!c) return;
-  void *bufs[] = {c->X, c->Y, c->inMean, c->inScale, c->outMean, c->outScale, c->corr, c->elemMass, c->shift};
+  void *bufs[] = {c->X,    c->Y,     c->inMean, c->inScale, c->outMean,   c->outScale,
+                  c->corr, c->elemMass, c->shift, c->dY,      c->dX,        c->vjpPartial};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   delete c;
@@ -1359,6 +1362,8 @@ int selfamd_model2d_destroy(selfamd_model2d *m) {
                     m->aux,  m->stat2, m->presTrace, m->ndsdx, m->nJ,    m->nnHat, m->nnScale};
   if (m->bandElems) (void)hipFree(m->bandElems);
   if (m->intScratch) (void)hipFree(m->intScratch);
+  if (m->adjTrace) (void)hipFree(m->adjTrace);
+  if (m->adjChi) (void)hipFree(m->adjChi);
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (m->nbr) (void)hipFree(m->nbr);
@@ -1676,7 +1681,9 @@ int selfamd_model2d_set_closure(selfamd_model2d *m, selfamd_mlp *mlp, const floa
             hipMalloc(&c->inMean, 4 * nIn) == hipSuccess && hipMalloc(&c->inScale, 4 * nIn) == hipSuccess &&
             hipMalloc(&c->outMean, 4 * nOut) == hipSuccess && hipMalloc(&c->outScale, 4 * nOut) == hipSuccess &&
             hipMalloc(&c->corr, 8 * nEl * nOut) == hipSuccess && hipMalloc(&c->elemMass, 8 * nEl) == hipSuccess &&
-            hipMalloc(&c->shift, 8) == hipSuccess;
+            hipMalloc(&c->shift, 8) == hipSuccess && hipMalloc(&c->dY, 4 * nEl * nOut) == hipSuccess &&
+            hipMalloc(&c->dX, 4 * nEl * nIn) == hipSuccess &&
+            hipMalloc(&c->vjpPartial, 8 * closure_vjp_partials(m->N, m->nEl)) == hipSuccess;
   ok = ok && hipMemcpy(c->inMean, inMean, 4 * nIn, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(c->inScale, inScale, 4 * nIn, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(c->outMean, outMean, 4 * nOut, hipMemcpyHostToDevice) == hipSuccess &&
@@ -1687,6 +1694,52 @@ int selfamd_model2d_set_closure(selfamd_model2d *m, selfamd_mlp *mlp, const floa
     return kHipError;
   }
   m->closure = c;
+  return kOk;
+}
+
+int selfamd_model2d_stage(selfamd_model2d *m, int stage, double dt, double *Ssave, double *unSave) {
+  SELFAMD_REQUIRE(m && stage >= 0 && stage < 3 && dt > 0.0, "model2d_stage: bad arguments (stage=%d dt=%g)", stage, dt);
+  SELFAMD_REQUIRE(m->nBands == 0, "model2d_stage: banded stepping is on");
+  const size_t nS = (size_t)(m->N + 1) * (m->N + 1) * m->nVar * m->nEl;
+  if (Ssave) SELFAMD_HIP(hipMemcpyAsync(Ssave, m->S, sizeof(double) * nS, hipMemcpyDeviceToDevice, m->stream));
+  if (unSave) {
+    SELFAMD_REQUIRE(m->un[0], "model2d_stage: this model has no lagged velocities to save");
+    const size_t nU = (size_t)(m->nodal ? 2 : 1) * (m->N + 1) * 4 * m->nEl;
+    SELFAMD_HIP(hipMemcpyAsync(unSave, m->un[m->cur], sizeof(double) * nU, hipMemcpyDeviceToDevice, m->stream));
+  }
+  SELFAMD_HIP(m->launch_stage(0, stage, dt, m->cur, nullptr));
+  m->cur ^= 1;
+  if (stage == 2) m->t += dt;
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+int selfamd_model2d_tendency_vjp(selfamd_model2d *m, const double *S, const double *un, const double *lambda,
+                                 double *mu, const double *Sprev, double *muPrev) {
+  SELFAMD_REQUIRE(m && S && un && lambda && mu, "model2d_tendency_vjp: null argument");
+  SELFAMD_REQUIRE(!Sprev == !muPrev, "model2d_tendency_vjp: Sprev and muPrev go together");
+  SELFAMD_REQUIRE(m->kind == SELFAMD_MODEL_SW && !m->nodal && !m->anyBoundary,
+                  "model2d_tendency_vjp: only nonlinear shallow water on affine meshes without boundary sides");
+  const size_t nTr = (size_t)(m->N + 1) * 3 * 4 * m->nEl;
+  // scratch: traces of S and of Sprev, neighbour-trace cotangents, lagged-velocity cotangents
+  if (!m->adjTrace) SELFAMD_HIP(hipMalloc(&m->adjTrace, sizeof(double) * 2 * nTr));
+  if (!m->adjChi) SELFAMD_HIP(hipMalloc(&m->adjChi, sizeof(double) * nTr * 5 / 3));
+  SELFAMD_HIP(Launcher<PhysSW>::traces(m->N, S, m->adjTrace, m->mats, m->p, m->nEl, m->stream));
+  if (Sprev) SELFAMD_HIP(Launcher<PhysSW>::traces(m->N, Sprev, m->adjTrace + nTr, m->mats, m->p, m->nEl, m->stream));
+  SELFAMD_HIP(sw_tendency_vjp(m->N, m->nEl, S, m->adjTrace, un, m->aux, m->geom, m->nbr, m->mats, m->p.g, m->p.fCori,
+                              m->p.auxConst, lambda, mu, m->adjChi, m->adjTrace + nTr, muPrev, m->adjChi + nTr,
+                              m->stream));
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
+  return kOk;
+}
+
+int selfamd_model2d_closure_vjp(selfamd_model2d *m, const double *S, const double *lambda, double *mu) {
+  SELFAMD_REQUIRE(m && S && lambda && mu, "model2d_closure_vjp: null argument");
+  SELFAMD_REQUIRE(m->closure, "model2d_closure_vjp: no closure attached");
+  const int P = m->N + 1;
+  SELFAMD_HIP(closure_vjp(*m->closure, S, m->geom, m->mats + 2 * P * P + 2 * P, lambda, mu, m->N, m->nVar, 2, m->nEl,
+                          m->stream));
+  SELFAMD_HIP(hipStreamSynchronize(m->stream));
   return kOk;
 }
 
