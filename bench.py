@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble", "closure"])
     ap.add_argument("--closure-runs", type=int, default=8, help="fine training runs of the closure workload")
+    ap.add_argument("--closure-aposteriori", type=int, default=1000,
+                    help="steps of training through the solver (closure workload; 0: a-priori only)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -390,7 +392,8 @@ def run_closure(a, root):
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
-    r = closure.benchmark(n_train=a.closure_runs, n_steps=a.steps or 500, log=log)
+    r = closure.benchmark(n_train=a.closure_runs, n_steps=a.steps or 500, log=log,
+                          aposteriori_iters=a.closure_aposteriori)
     runs = r["runs"]
     value = float(np.mean([x["speedup_at_equal_error"] for x in runs]))
     out = {
@@ -402,7 +405,8 @@ def run_closure(a, root):
                                "GELU, trained on fine 256x256 N=7 runs projected onto the coarse space",
                    "train_runs": a.closure_runs, "train_samples": r["train_samples"], "alpha": r["alpha"],
                    "skill_train": r["skill_train"], "skill_validation": r["skill_validation"],
-                   "data_s": r["data_s"], "train_s": r["train_s"], "runs": runs},
+                   "data_s": r["data_s"], "train_s": r["train_s"], "runs": runs,
+                   "aposteriori": r.get("aposteriori")},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out, default=float), flush=True)

@@ -416,7 +416,34 @@ def plain_ladder(cf, seed, n_steps, factors=(1, 2, 4), log=None):
     return out, fine_ms, sol0
 
 
-def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, seeds_eval=(1000, 1001), log=None):
+def aposteriori_network(cf, windows, iters, lr=1e-3, K=None, seed=0, log=None):
+    """A network trained through the solver from a zero correction: normalisation of the
+    inputs from the window states and of the outputs from the one-step defects of the windows,
+    last layer zero (the untrained network is the plain run), then train_aposteriori."""
+    X = np.concatenate([cf.raw_features(w[1]) for w in windows])
+    T = []
+    models = {}
+    for w in windows:
+        key = id(w[0])
+        if key not in models:
+            models[key] = cf.coarse_model(w[0])
+        C = models[key]
+        C.SetSolution(w[1])
+        C.ForwardStep(1, cf.dt_c)
+        T.append(((w[2][0] - C.solution) / cf.dt_c).reshape(w[1].shape[0], -1))
+    norm = Normalisation(X, np.concatenate(T))
+    norm.outMean[:] = 0.0
+    mlp = MLP(cf.nIn, 128, cf.nOut, max(4096, cf.nxc * cf.nxc))
+    p = init_params(cf.nIn, 128, cf.nOut, seed=seed)
+    p["W3"][:] = 0.0
+    p["b3"][:] = 0.0
+    mlp.set_params(p)
+    hist = train_aposteriori(cf, mlp, norm, windows, iters, lr=lr, lr_final=lr / 10, seed=seed, log=log)
+    return mlp, norm, hist
+
+
+def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, seeds_eval=(1000, 1001), log=None,
+              aposteriori_iters=0, aposteriori_K=8):
     """Closure workload: train on fine runs, pick the correction scale on a validation run,
     then compare on held-out runs the corrected coarse run with plain runs of increasing
     resolution.  'speedup_at_equal_error' is the wall-time of the cheapest plain run whose
@@ -451,4 +478,28 @@ def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, se
                      "closure_mass_drift": c["mass_drift"], "speedup_at_equal_error": ref_ms / ms,
                      "speedup_vs_fine": fine_ms / ms})
     res["runs"] = runs
+    if aposteriori_iters:
+        # training through the solver: windows of the training runs, network from a zero correction
+        t2 = time.perf_counter()
+        wins = rollout_windows(cf, range(n_train), aposteriori_K, every=25, n_steps=n_steps, log=log)
+        mlp_a, norm_a, hist_a = aposteriori_network(cf, wins, aposteriori_iters, K=aposteriori_K, log=log)
+        weights = l2_weights(cf)
+        val = rollout_windows(cf, [600], aposteriori_K, every=50, n_steps=n_steps)
+        plain_l, corr_l = [], []
+        for w in val:
+            C = cf.coarse_model(w[0])
+            C.SetClosure(mlp_a, *norm_a.arrays(0.0))
+            plain_l.append(rollout_loss_grad(C, mlp_a, w[1], w[2], cf.dt_c, weights, want_grad=False))
+            C.SetClosure(mlp_a, *norm_a.arrays(1.0))
+            corr_l.append(rollout_loss_grad(C, mlp_a, w[1], w[2], cf.dt_c, weights, want_grad=False))
+        ap = {"windows": len(wins), "K": aposteriori_K, "iters": aposteriori_iters, "train_s": time.perf_counter() - t2,
+              "loss_first": float(np.mean(hist_a[:50])), "loss_last": float(np.mean(hist_a[-50:])),
+              "validation_window_loss_plain": float(np.mean(plain_l)),
+              "validation_window_loss_closure": float(np.mean(corr_l)), "runs": []}
+        for seed in seeds_eval:
+            r = evaluate(cf, seed, mlp_a, norm_a, n_steps=n_steps, alphas=(1.0,), log=log)
+            ap["runs"].append({"seed": seed, "plain_err": r["plain_err"][-1], "closure_err": r["closure"][1.0]["err"][-1],
+                               "closure_mass_drift": r["closure"][1.0]["mass_drift"],
+                               "error_ratio": r["closure"][1.0]["err"][-1] / r["plain_err"][-1]})
+        res["aposteriori"] = ap
     return res

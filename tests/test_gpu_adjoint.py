@@ -153,3 +153,25 @@ def test_rollout_gradient_matches_finite_differences():
     assert np.isfinite(loss) and loss > 0
     # measured 1.110356e-7 against 1.110369e-7 (h = 3e-3) and 1.110355e-7 (h = 1e-3)
     assert abs(adj - fd) <= 1e-3 * abs(fd), (adj, fd)
+
+
+def test_aposteriori_training_reduces_the_window_loss():
+    """Training through the solver on one window of a small configuration (coarse 16 x 16 N = 3,
+    fine 64 x 64 N = 7): from the zero correction (the plain run) the rollout loss of that
+    window goes down, and the corrected run conserves mass."""
+    from self_amd import closure as cl
+    cf = cl.CoarseFine(nxc=16)
+    wins = cl.rollout_windows(cf, [0], 4, every=25, n_steps=50)
+    w = wins[1]
+    mlp, norm, hist = cl.aposteriori_network(cf, [w], 150, lr=1e-3)
+    weights = cl.l2_weights(cf)
+    C = cf.coarse_model(w[0])
+    C.SetClosure(mlp, *norm.arrays(0.0))
+    plain = cl.rollout_loss_grad(C, mlp, w[1], w[2], cf.dt_c, weights, want_grad=False)
+    C.SetClosure(mlp, *norm.arrays(1.0))
+    trained = cl.rollout_loss_grad(C, mlp, w[1], w[2], cf.dt_c, weights, want_grad=False)
+    print(f"window loss plain {plain:.6e} trained {trained:.6e} ratio {trained / plain:.4f}", flush=True)
+    assert np.isfinite(hist).all()
+    assert trained < plain
+    m0 = cf.mass(w[1])
+    assert abs(cf.mass(C.solution) - m0) <= 1e-13 * abs(m0)
