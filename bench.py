@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nodal-metrics", action="store_true",
+                    help="run the engine on the per-node metric terms (the reference's representation) "
+                         "instead of one affine record per element")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for HBM counter runs: 3 dispatches of a 1 GiB FP64 copy (calibration), "
                          "then 2 RK3 steps (6 stage launches) of the workload; prints no JSON")
@@ -249,8 +252,9 @@ def run_single(a, root):
     from self_amd import workloads
     w = setup_single(self_amd, workloads, a.workload, a.seed)
     model, dt, N = w["model"], w["dt"], w["N"]
-    # the benchmark meshes are affine: one metric record per element (raises if a mesh is not)
-    model.SetNodalMetrics(False)
+    # the benchmark meshes are affine: one metric record per element (raises if a mesh is not),
+    # unless the reference's per-node metric terms are asked for
+    model.SetNodalMetrics(bool(a.nodal_metrics))
     if a.traffic_probe:
         return traffic_probe(self_amd, model, dt)
     steps = a.steps or w["steps"]
@@ -287,8 +291,12 @@ def run_single(a, root):
         cpu = cpu_single(oracle, a.workload, w, steps_all, steps_one)
     phys, kind = {"sw256": ("PhysSW", 2), "euler512": ("PhysEuler", 3), "lsw10": ("PhysLSW", 1)}[a.workload]
     kernel = tile_name(self_amd, N, phys, kind)
-    traffic = traffic_from_profiles(root, {"sw256": "sw2d_stage_256x256_N7_traffic.json",
-                                           "euler512": "euler2d_stage_512x512_N7_traffic.json"}.get(a.workload, ""))
+    if a.nodal_metrics:
+        t = {7: 64, 6: 64, 5: 128, 4: 128}.get(N, 256)
+        kernel = f"k_fused<{phys}, {N}, 0, true, {t}, 1, false>"
+    traffic = None if a.nodal_metrics else traffic_from_profiles(
+        root, {"sw256": "sw2d_stage_256x256_N7_traffic.json",
+               "euler512": "euler2d_stage_512x512_N7_traffic.json"}.get(a.workload, ""))
     copy_gbs = copy_bandwidth(self_amd)
     out = {
         "metric": METRIC if a.workload == "sw256" else f"DOF-updates/sec + % HBM roofline, {a.workload}",
@@ -296,7 +304,8 @@ def run_single(a, root):
         "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall,
-                   "metrics": "affine, one record per element",
+                   "metrics": ("per-node metric terms (SEMQuad)" if a.nodal_metrics
+                               else "affine, one record per element"),
                    "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB queued ahead of the warm-up steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,

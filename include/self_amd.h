@@ -258,7 +258,13 @@ typedef struct selfamd_model2d selfamd_model2d;
 #define SELFAMD_PARAM_GAMMA 3  /* Euler expansion factor Cp/Cv */
 #define SELFAMD_PARAM_RGAS 4   /* Euler gas constant */
 #define SELFAMD_PARAM_NODAL_METRICS 5  /* 1: per-node metric terms (as SEMQuad stores them);
-                                          0: one affine record per element (affine meshes) */
+                                          0: one affine record per element (affine meshes).
+                                          The affine record takes node (0,0) and side-node-0
+                                          values, so it departs from the reference's per-node
+                                          metrics by their round-off (~1e-12 relative on
+                                          affine meshes), which moves the tendency by up to
+                                          ~1e-10 relative (7e-11 at 256^2 on the unit square):
+                                          set 1 for the reference's arithmetic to 1e-12 */
 /* static per-node fields (selfamd_model2d_set_field), interior(0:N,0:N,1,nEl) device arrays */
 #define SELFAMD_FIELD_TOPOGRAPHY 1  /* SW this%H (SetTopography) */
 #define SELFAMD_FIELD_BATHYMETRY 2  /* LSW this%H (SetBathymetry) */
@@ -269,8 +275,9 @@ typedef struct selfamd_model2d selfamd_model2d;
 /* Init_Model2D + the model's Init (e.g. src/models/SELF_ShallowWater.f90:96-147).  sideInfo is
  * host memory; geometry arrays are SEMQuad device arrays (dsdx interior, J interior, nHat and
  * nScale boundary) from selfamd_semquad_generate, copied into the engine.  Affine meshes
- * (metric terms constant per element) default to one metric record per element; other
- * meshes use the per-node metrics (see SELFAMD_PARAM_NODAL_METRICS). */
+ * (metric terms constant per element to 1e-10 relative) default to one metric record per
+ * element, a representation that departs from the reference's per-node metric values at the
+ * 1e-10 level (see SELFAMD_PARAM_NODAL_METRICS); other meshes use the per-node metrics. */
 int selfamd_model2d_create(selfamd_model2d **model, int kind, int N, int nEl, const int32_t *sideInfo,
                            const double *dsdx_dev, const double *J_dev, const double *nHatb_dev,
                            const double *nScaleb_dev, void *stream);

@@ -117,7 +117,7 @@ struct PhysSW {
     src[2] = 0.0;
   }
   // SetBoundaryCondition_ShallowWater (src/models/SELF_ShallowWater.f90:284-343)
-  __device__ static void bc(const double *qL, double auxB, double nx, double ny, int bcid, const double *, double *qR) {
+  __device__ static void bc(const double *qL, double auxB, double nx, double ny, int bcid, const double *, bool, double *qR) {
     if (bcid == SELFAMD_BC_NONORMALFLOW) {
       qR[0] = (ny * ny - nx * nx) * qL[0] - 2.0 * nx * ny * qL[1];
       qR[1] = (nx * nx - ny * ny) * qL[1] - 2.0 * nx * ny * qL[0];
@@ -163,7 +163,7 @@ struct PhysLSW {
     src[1] = -f * s[0];
     src[2] = 0.0;
   }
-  __device__ static void bc(const double *qL, double, double nx, double ny, int bcid, const double *, double *qR) {
+  __device__ static void bc(const double *qL, double, double nx, double ny, int bcid, const double *, bool, double *qR) {
     if (bcid == SELFAMD_BC_NONORMALFLOW) {
       qR[0] = (ny * ny - nx * nx) * qL[0] - 2.0 * nx * ny * qL[1];
       qR[1] = (nx * nx - ny * ny) * qL[1] - 2.0 * nx * ny * qL[0];
@@ -225,7 +225,8 @@ struct PhysEuler {
     src[3] = -s[0] * gx - s[0] * gy;  // quirk: energy source uses rho*u twice
   }
   // SetBoundaryCondition_CompressibleIdealGas2D (src/models/SELF_CompressibleIdealGas2D.f90:1217-1323)
-  __device__ static void bc(const double *qL, double, double nx, double ny, int bcid, const double *pres, double *qR) {
+  __device__ static void bc(const double *qL, double, double nx, double ny, int bcid, const double *pres, bool hasPres,
+                            double *qR) {
     if (bcid == SELFAMD_BC_NONORMALFLOW) {
       qR[0] = (ny * ny - nx * nx) * qL[0] - 2.0 * nx * ny * qL[1];
       qR[1] = (nx * nx - ny * ny) * qL[1] - 2.0 * nx * ny * qL[0];
@@ -233,7 +234,7 @@ struct PhysEuler {
       qR[4] = (ny * ny - nx * nx) * qL[4] - 2.0 * nx * ny * qL[5];
       qR[5] = (nx * nx - ny * ny) * qL[5] - 2.0 * nx * ny * qL[4];
       qR[6] = qL[6]; qR[7] = qL[7]; qR[8] = qL[8];
-    } else if ((bcid == SELFAMD_BC_PRESCRIBED || bcid == SELFAMD_BC_RADIATION) && pres) {
+    } else if ((bcid == SELFAMD_BC_PRESCRIBED || bcid == SELFAMD_BC_RADIATION) && hasPres) {
       for (int k = 0; k < 9; k++) qR[k] = pres[k];
     } else {
       for (int k = 0; k < 9; k++) qR[k] = 0.0;
@@ -550,12 +551,12 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       } else {
         const double *pres =
             a.presTrace ? a.presTrace + (size_t)fi_ + NP * (size_t)NT * (side_ + 4 * (size_t)fe_) : nullptr;
+        // the prescribed values go by value with a flag (a pointer that may be null would keep
+        // the array in scratch memory)
         double pv[NT];
-        if (pres) {
 #pragma unroll
-          for (int k = 0; k < NT; k++) pv[k] = pres[(size_t)k * NP];
-        }
-        Phys::bc(qL, auxB, nx_, ny_, -code_ - 1, pres ? pv : nullptr, qR_);
+        for (int k = 0; k < NT; k++) pv[k] = pres ? pres[(size_t)k * NP] : 0.0;
+        Phys::bc(qL, auxB, nx_, ny_, -code_ - 1, pv, pres != nullptr, qR_);
       }
     }
     if (Phys::kLag) {
