@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--closure-aposteriori", type=int, default=1000,
                     help="steps of training through the solver (closure workload; 0: a-priori only)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--batch-mode", default="per-group", choices=["per-group", "persistent"],
+                    help="ensemble: one launch per degree group and stage, or one persistent launch per stage")
     ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nodal-metrics", action="store_true",
@@ -348,6 +350,9 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
             print(f"ensemble: {r + 1}/{n_runs} runs set up", file=sys.stderr, flush=True)
     setup_s = time.perf_counter() - t0
     batch = self_amd.Batch(models, dts)
+    persistent = a.batch_mode == "persistent"
+    if persistent:
+        batch.SetMode(self_amd.Batch.PERSISTENT)
     groups, dof = batch.info()
     steps = a.steps or 20
     # the batch runs on its own stream: the settling copies run beside the warm-up steps on the
@@ -380,12 +385,15 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
         "unit": "DOF-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n_runs} nonlinear SW 2D runs, n x n periodic elements with n ~ U{{8..64}}, "
-                               f"N ~ U{{3,5,7}}, one fused launch per degree group and RK stage, one hipGraph "
-                               f"per step", "runs": n_runs, "groups": groups, "dof": dof, "setup_s": setup_s,
+                               f"N ~ U{{3,5,7}}, "
+                               + ("one persistent launch per RK stage for all degree groups"
+                                  if persistent else "one fused launch per degree group and RK stage")
+                               + ", one hipGraph per step", "batch_mode": a.batch_mode, "runs": n_runs, "groups": groups, "dof": dof, "setup_s": setup_s,
                    "max_rel_err_vs_separate": max(errs), "runs_checked": n_check},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused<PhysSW, N, 0, false, T> (one per degree group)"},
+                     "kernel": "k_batch_persistent<PhysSW>" if persistent
+                     else "k_fused<PhysSW, N, 0, false, T> (one per degree group)"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -371,6 +371,16 @@ int selfamd_batch_time_steps(selfamd_batch *batch, int nSteps, double *elapsedMs
 int selfamd_batch_scatter(selfamd_batch *batch);
 int selfamd_batch_info(selfamd_batch *batch, int *nGroups, double *totalDof);
 int selfamd_batch_destroy(selfamd_batch *batch);
+/* Launch structure of the batch's RK3 stages (no reference counterpart):
+ * SELFAMD_BATCH_PER_GROUP (default) one fused launch per degree group and stage, each with its
+ * degree's tuned tile; SELFAMD_BATCH_PERSISTENT one launch per stage for all groups, a
+ * persistent grid (the device's CUs times the resident blocks per CU) whose blocks stride
+ * through every group's 256-wide tiles, group after group (nonlinear SW, degrees 3, 5 and 7;
+ * refused when the tuned N = 5 tile is 64 wide).  Either way each run is bit-identical to its
+ * separate solve.  Synchronizes the batch's stream and recaptures the step graphs. */
+#define SELFAMD_BATCH_PER_GROUP 0
+#define SELFAMD_BATCH_PERSISTENT 1
+int selfamd_batch_set_mode(selfamd_batch *batch, int mode);
 
 /* ------------------------------------------------------------------------------------ */
 /* Learned subgrid closure: FP32 MLP, fine -> coarse projection, model hook               */

@@ -595,7 +595,13 @@ class Batch:
     """Many independent models advanced together (no reference counterpart).
 
     Runs of one polynomial degree share one fused launch per RK stage and every degree group
-    of a step sits in one hipGraph.  `scatter()` hands each run's state back to its model."""
+    of a step sits in one hipGraph.  `SetMode(Batch.PERSISTENT)` instead advances all groups
+    (nonlinear SW, degrees 3, 5, 7) in one persistent launch per stage; both modes give each
+    run its separate solve's results bit for bit.  `scatter()` hands each run's state back to
+    its model."""
+
+    PER_GROUP = 0
+    PERSISTENT = 1
 
     def __init__(self, models, dts):
         assert len(models) == len(dts) and models
@@ -604,6 +610,9 @@ class Batch:
         dt = np.ascontiguousarray(dts, dtype=np.float64)
         self._h = ctypes.c_void_p()
         call("selfamd_batch_create", ctypes.byref(self._h), len(models), arr, _ptr(dt))
+
+    def SetMode(self, mode):
+        call("selfamd_batch_set_mode", self._h, int(mode))
 
     def ForwardStep(self, nSteps):
         call("selfamd_batch_forward_step", self._h, int(nSteps))

@@ -97,6 +97,7 @@ SIGNATURES = {
     "selfamd_batch_scatter": [c_void_p],
     "selfamd_batch_info": [c_void_p, c_void_p, c_void_p],
     "selfamd_batch_destroy": [c_void_p],
+    "selfamd_batch_set_mode": [c_void_p, c_int],
     "selfamd_mlp_create": [c_void_p, c_int, c_int, c_int, c_int],
     "selfamd_mlp_destroy": [c_void_p],
     "selfamd_mlp_num_params": [c_void_p, c_void_p],

@@ -320,53 +320,93 @@ __device__ inline double pair_sum32(double x) {
 //
 // Tile parameters: THREADS work-items per block (THREADS / (N+1)^2 elements), WPE the
 // occupancy the register allocator must allow (waves per SIMD; 1 = unconstrained), XCD an
-// element order in which the blocks one XCD receives (the dispatcher deals blocks to the 8
-// XCDs in turn) cover a contiguous run of elements, so that neighbours share that XCD's L2.
+// element order in which the blocks one XCD receives cover a contiguous run of elements.
 // With a.reverse the elements are visited last to first: successive stages alternate
-// direction, so a stage starts on the elements whose data the previous stage touched last
-// and which are still held by the memory-side Infinity Cache.
-template <class Phys, int N, int MODE, bool NODAL, int THREADS, int WPE = 1, bool XCD = false>
-__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_fused(const FusedArgs a) {
+// direction, so a stage starts on the elements whose data the previous stage touched last.
+//
+// LDS of one tile (block) of the stage kernel, in doubles: the node arrays sQ, sF1, sF2, the
+// static field sAux, the nodal-mode metric products sGb, the face fluxes sBF and the matrices
+// sM, each starting on a 16-byte boundary.
+template <class Phys, int N, bool NODAL, int THREADS>
+struct TileLds {
+  static constexpr int NP = N + 1, NN = NP * NP, EPB = THREADS / NN, NF = 4 * NP;
+  static constexpr int NV = Phys::NV, NT = Phys::NT;
+  static constexpr bool USE_AUX = Phys::kAuxGrad || Phys::kAuxTrace;
+  static constexpr bool GRADN = NODAL && Phys::kAuxGrad;
+  // At N = 5 two pad doubles per element array (38 instead of 36) shift the elements of a
+  // tile against each other in the banks: 70.0 against 71.1 us per SW launch at 256^2 (row
+  // strides of 7 were slower, 72.3-72.7 us).  Matrices hold entry (c, r) at c + MRS r, with two
+  // pad doubles per 8-double row (they are written once per tile).
+  static constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NP == 6 ? NN + 2 : NN;
+  static constexpr int even(int n) { return (n + 1) & ~1; }
+  static constexpr int Q = 0;
+  static constexpr int F1 = Q + even(NT * EPB * NNS);
+  static constexpr int F2 = F1 + even(NV * EPB * NNS);
+  static constexpr int AUX = F2 + even(NV * EPB * NNS);
+  static constexpr int GB = AUX + even(USE_AUX ? EPB * NNS : 1);
+  static constexpr int BF = GB + even(GRADN ? 4 * EPB * NNS : 1);
+  static constexpr int M = BF + even(NV * EPB * NF);
+  static constexpr int SIZE = M + even(2 * NP * MRS + 4 * NP);
+  // face-lane arrangement (see the stage kernel): every face node has its own lane; two lanes per
+  // face node; N = 5 face lanes padded to 32 per element; every element's node and face lanes in
+  // one wavefront
+  static constexpr bool ONE_FACE_PASS = EPB * NF <= THREADS;
+  static constexpr bool SPLIT = ONE_FACE_PASS && 2 * EPB * NF <= THREADS && NF <= 32;
+  static constexpr bool FPAD = !SPLIT && NF == 24 && 32 * EPB <= THREADS;
+  static constexpr bool WAVE = (NN == 64 && SPLIT) || (NN == 16 && NF == 16 && ONE_FACE_PASS && !SPLIT && !FPAD);
+};
+
+// The tile's interpolation and derivative matrices and quadrature weights into its LDS (dgM, D
+// in rows of MRS, bM(2P), qW, 1/qW), from mats = [dgM, D, bM, qW, 1/qW] as SetupMatrices stores
+// them.  Every work-item of the block takes part.
+template <class Phys, int N, bool NODAL, int THREADS>
+__device__ __forceinline__ void load_tile_matrices(const double *mats, double *lds, int tid) {
+  using L = TileLds<Phys, N, NODAL, THREADS>;
+  constexpr int NP = N + 1, NN = NP * NP, MRS = L::MRS;
+  double *sM = lds + L::M;
+  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) {
+    const int r = k % NN;
+    sM[k < 2 * NN ? (k / NN) * NP * MRS + (r / NP) * MRS + r % NP : 2 * NP * MRS + k - 2 * NN] = mats[k];
+  }
+}
+
+// One tile of the stage kernel: tile blk of the launch's element list (after any reordering),
+// with the tile's LDS at lds, run by work-item tid of the block.  The __global__ wrappers
+// below map blocks to tiles.  PRELOADED: the block has already put the matrices into the LDS
+// (load_tile_matrices) and made them visible, so the tile neither loads them nor needs a block
+// barrier where the element's own wavefront suffices.
+template <class Phys, int N, int MODE, bool NODAL, int THREADS, bool PRELOADED = false>
+__device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *lds, const int tid) {
+  using L = TileLds<Phys, N, NODAL, THREADS>;
   constexpr int NP = N + 1, NN = NP * NP, EPB = THREADS / NN, NF = 4 * NP;
   constexpr int NV = Phys::NV, NT = Phys::NT;
   constexpr bool USE_AUX = Phys::kAuxGrad || Phys::kAuxTrace;
-  constexpr bool ONE_FACE_PASS = EPB * NF <= THREADS;  // every face node has its own lane
+  constexpr bool ONE_FACE_PASS = L::ONE_FACE_PASS;  // every face node has its own lane
   // two lanes per face node (lanes l and l ^ 32 of a wavefront): each interpolates half of
   // the face line, the halves are summed across the pair
-  constexpr bool SPLIT = ONE_FACE_PASS && 2 * EPB * NF <= THREADS && NF <= 32;
+  constexpr bool SPLIT = L::SPLIT;
   static_assert(EPB >= 1, "block too small for one element");
   // LDS layouts.  Node arrays hold node (i, j) at slot(i, j).  With 8 x 8 nodes a row is 64
   // bytes, and a column access (fixed i, one row per lane group) would find rows 0, 2, 4, 6 in
   // the same banks; permuting the columns of row j by i ^ (j & 6) spreads them over the banks,
   // keeps the pairs (2m, 2m + 1) adjacent for 16-byte reads and keeps every row in place, so
-  // row-wise writes stay conflict-free.  Matrices hold entry (c, r) at c + MRS r, with two pad
-  // doubles per 8-double row for the same reason (they are written once per block).
+  // row-wise writes stay conflict-free.
   constexpr bool SWZ = NP == 8;
-  // At N = 5 two pad doubles per element array (38 instead of 36) shift the elements of a
-  // tile against each other in the banks: 70.0 against 71.1 us per SW launch at 256^2 (row
-  // strides of 7 were slower, 72.3-72.7 us).
-  constexpr int MRS = NP % 8 == 0 ? NP + 2 : NP, NNS = NP == 6 ? NN + 2 : NN;
+  constexpr int MRS = L::MRS, NNS = L::NNS;
   auto slot = [](int i_, int j_) { return SWZ ? NP * j_ + (i_ ^ (j_ & 6)) : i_ + NP * j_; };
-  __shared__ __align__(16) double sQ[NT][EPB][NNS];
-  __shared__ __align__(16) double sF1[NV][EPB][NNS];
-  __shared__ __align__(16) double sF2[NV][EPB][NNS];
-  __shared__ __align__(16) double sAux[USE_AUX ? EPB : 1][USE_AUX ? NNS : 1];
+  auto sQ = reinterpret_cast<double (*)[EPB][NNS]>(lds + L::Q);
+  auto sF1 = reinterpret_cast<double (*)[EPB][NNS]>(lds + L::F1);
+  auto sF2 = reinterpret_cast<double (*)[EPB][NNS]>(lds + L::F2);
+  auto sAux = reinterpret_cast<double (*)[USE_AUX ? NNS : 1]>(lds + L::AUX);
   // nodal mode: static field times the node's contravariant metric, for GradientSF
-  constexpr bool GRADN = NODAL && Phys::kAuxGrad;
-  __shared__ double sGb[GRADN ? 4 : 1][GRADN ? EPB : 1][GRADN ? NNS : 1];
-  __shared__ double sBF[NV][EPB][NF];
-  __shared__ __align__(16) double sM[2 * NP * MRS + 4 * NP];  // dgM, D (rows of MRS), bM(2P), qW, 1/qW
+  constexpr bool GRADN = L::GRADN;
+  auto sGb = reinterpret_cast<double (*)[GRADN ? EPB : 1][GRADN ? NNS : 1]>(lds + L::GB);
+  auto sBF = reinterpret_cast<double (*)[EPB][NF]>(lds + L::BF);
+  double *sM = lds + L::M;  // dgM, D (rows of MRS), bM(2P), qW, 1/qW
 
-  const int tid = threadIdx.x;
   const int el = tid / NN, node = tid % NN;
   const int i = node % NP, j = node / NP;
   const int nl = slot(i, j);  // the node's LDS slot
-  int blk = blockIdx.x;
-  if constexpr (XCD) {
-    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, x = blk & 7, k = blk >> 3;
-    blk = x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
-  }
-  if (a.reverse) blk = (int)gridDim.x - 1 - blk;
   const int pe = blk * EPB + el;  // position in the launch's element list
   bool nodeActive = el < EPB && pe < a.nEl;
   int e = a.elems ? (nodeActive ? a.elems[pe] : 0) : pe;
@@ -416,7 +456,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // FPAD (N = 5, tiles of 128 or 256): each element's 24 face nodes take 32 lanes, south and
   // north faces in lanes 0-11 and east and west in lanes 16-27 of the group, so that a 16-lane
   // LDS group reads either columns or rows of the element, never both (no bank conflicts)
-  constexpr bool FPAD = !SPLIT && NF == 24 && 32 * EPB <= THREADS;
+  constexpr bool FPAD = L::FPAD;
   const int fhalf = SPLIT ? (tid >> 5) & 1 : 0;
   const int fslot = SPLIT ? (tid & 31) : FPAD ? (tid & 15) : 0;
   const int fidx = SPLIT ? (tid >> 6) * NF + fslot : FPAD ? (tid >> 5) * NF + (fslot % 12) : tid;
@@ -431,7 +471,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
   // element-to-wavefront locality of the node and face lanes (see phase_sync): at N = 7 a
   // wavefront owns one element (face lanes split in pairs); at N = 3 an element's 16 nodes and 16
   // face nodes share the same 16 lanes
-  constexpr bool WAVE = (NN == 64 && SPLIT) || (NN == 16 && NF == 16 && ONE_FACE_PASS && !SPLIT && !FPAD);
+  constexpr bool WAVE = L::WAVE;
   const int pf = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && (!FPAD || fslot < 2 * NP) && fel < EPB &&
                           pf < a.nEl;
@@ -472,10 +512,7 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
     }
   }
-  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) {
-    const int r = k % NN;
-    sM[k < 2 * NN ? (k / NN) * NP * MRS + (r / NP) * MRS + r % NP : 2 * NP * MRS + k - 2 * NN] = a.mats[k];
-  }
+  if constexpr (!PRELOADED) load_tile_matrices<Phys, N, NODAL, THREADS>(a.mats, lds, tid);
   const double *dgM = sM, *Dm = sM + NP * MRS, *bM0 = sM + 2 * NP * MRS, *bM1 = bM0 + NP, *qW = bM1 + NP;
 
   // ---- phase 0: volume fluxes, nodal trace quantities ----
@@ -501,7 +538,9 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
       for (int k = 0; k < 4; k++) sGb[k][el][nl] = auxVal * dsdx[k];
     }
   }
-  __syncthreads();
+  // the matrices (written by the whole block) and the element's node values
+  if constexpr (PRELOADED) phase_sync<WAVE>();
+  else __syncthreads();
 
   // ---- phase 1: faces ----
   auto face = [&](int fel_, int side_, int fi_, int fe_, int fnode_, int code_, double *qR_, double unL_, double unR_,
@@ -787,6 +826,79 @@ __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WP
 #pragma unroll
     for (int k = 0; k < NT; k++) tr[(size_t)k * NP] = qn[k];
   }
+}
+
+// The stage kernel: one tile per block.  XCD orders the tiles so that the blocks one XCD
+// receives (the dispatcher deals blocks to the 8 XCDs in turn) cover a contiguous run of
+// elements, whose neighbours then share that XCD's L2; a.reverse visits the tiles last to
+// first (successive stages alternate direction).
+template <class Phys, int N, int MODE, bool NODAL, int THREADS, int WPE = 1, bool XCD = false>
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_fused(const FusedArgs a) {
+  __shared__ __align__(16) double lds[TileLds<Phys, N, NODAL, THREADS>::SIZE];
+  int blk = blockIdx.x;
+  if constexpr (XCD) {
+    const int nb = gridDim.x, q = nb >> 3, rem = nb & 7, x = blk & 7, k = blk >> 3;
+    blk = x < rem ? x * (q + 1) + k : rem * (q + 1) + (x - rem) * q + k;
+  }
+  if (a.reverse) blk = (int)gridDim.x - 1 - blk;
+  fused_tile<Phys, N, MODE, NODAL, THREADS>(a, blk, lds, threadIdx.x);
+}
+
+// Batched runs of several degrees in ONE launch per RK stage: a persistent grid (as many blocks
+// as the device holds at once) over the tile lists of the degree groups N = 3, 5, 7, in that
+// order (a tile of 256 work-items: 16, 7 or 4 elements).  Blocks are dispatched to the eight
+// XCDs in turn (block b to XCD b mod 8), so each group's tile list is cut into eight contiguous
+// parts and the blocks of XCD x stride through part x: every block gets the same mix of
+// degrees and all blocks sweep the groups together.  A block loads a group's matrices into its
+// LDS once; where each element lives in one wavefront (N = 3 and 7) the tiles then need no
+// block barrier at all.  Per tile the arithmetic is that of the degree's own stage kernel, so
+// every run stays bit-identical to its separate solve; each group's element order alternates
+// between stages.
+constexpr int kPersistentDegrees[3] = {3, 5, 7};
+struct PersistentArgs {
+  FusedArgs g[3];  // the groups of degree 3, 5, 7 (nEl = 0 when a degree is absent)
+};
+static_assert(sizeof(PersistentArgs) <= 4096, "kernel arguments exceed the 4 KiB kernarg segment");
+
+template <class Phys>
+__global__ void __launch_bounds__(256) k_batch_persistent(const PersistentArgs b) {
+  constexpr int S3 = TileLds<Phys, 3, false, 256>::SIZE, S5 = TileLds<Phys, 5, false, 256>::SIZE,
+                S7 = TileLds<Phys, 7, false, 256>::SIZE;
+  constexpr int SMAX = S3 > S5 ? (S3 > S7 ? S3 : S7) : (S5 > S7 ? S5 : S7);
+  __shared__ __align__(16) double lds[SMAX];
+  const int G = gridDim.x, x = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int cx = (G >> 3) + (x < (G & 7) ? 1 : 0);  // blocks on this XCD
+  // one loop per degree, so that what the compiler keeps in registers across one degree's
+  // tiles is not live during another's
+  auto group = [&](auto deg, int k) {
+    constexpr int N = decltype(deg)::value;
+    using L = TileLds<Phys, N, false, 256>;
+    const FusedArgs &a = b.g[k];
+    const int nt = (a.nEl + L::EPB - 1) / L::EPB;
+    const int l0 = (int)(((long long)nt * x) >> 3) + r, hi = (int)(((long long)nt * (x + 1)) >> 3);
+    if (l0 >= hi) return;  // uniform over the block
+    // the previous group's tiles are done with the LDS
+    __syncthreads();
+    load_tile_matrices<Phys, N, false, 256>(a.mats, lds, threadIdx.x);
+    __syncthreads();
+#pragma unroll 1
+    for (int l = l0; l < hi; l += cx) {
+      const int blk = a.reverse ? nt - 1 - l : l;
+      // the work-item index, opaque to the compiler in each iteration: the tile's lane-derived
+      // offsets are then formed per tile as in the one-tile kernel, not hoisted out of the loop
+      // and held in registers across it
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
+      __builtin_assume(tid >= 0 && tid < 256);
+      fused_tile<Phys, N, 0, false, 256, true>(a, blk, lds, tid);
+      // the next tile reuses the LDS: where each element lives in one wavefront, that
+      // wavefront's own accesses are in program order
+      phase_sync<L::WAVE>();
+    }
+  };
+  group(std::integral_constant<int, 3>{}, 0);
+  group(std::integral_constant<int, 5>{}, 1);
+  group(std::integral_constant<int, 7>{}, 2);
 }
 
 // ------------------------------------------------------------------------------
@@ -1911,7 +2023,12 @@ struct selfamd_batch {
   int cur = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 
-  hipError_t launch_group(BatchGroup &g, int stage, int tin) {
+  // SELFAMD_BATCH_PER_GROUP: one launch per degree group and stage; SELFAMD_BATCH_PERSISTENT:
+  // one persistent launch per stage for all groups (SW, degrees 3, 5, 7)
+  int mode = SELFAMD_BATCH_PER_GROUP;
+  int persistentBlocks = 0;
+
+  FusedArgs group_args(const BatchGroup &g, int stage, int tin) const {
     FusedArgs a{};
     a.S = g.S; a.G = g.G; a.dSdt = nullptr;
     a.traceIn = g.trace[tin]; a.traceOut = g.trace[tin ^ 1];
@@ -1923,11 +2040,73 @@ struct selfamd_batch {
     a.reverse = tile_alternates(kind, g.N, g.nEl) ? tin : 0;
     a.anyBoundary = anyBoundary;
     a.p = p;
+    return a;
+  }
+  hipError_t launch_group(BatchGroup &g, int stage, int tin) {
+    const FusedArgs a = group_args(g, stage, tin);
     switch (kind) {
       case SELFAMD_MODEL_SW: return Launcher<PhysSW>::stage(g.N, 0, false, a, stream);
       case SELFAMD_MODEL_LSW: return Launcher<PhysLSW>::stage(g.N, 0, false, a, stream);
       default: return Launcher<PhysEuler>::stage(g.N, 0, false, a, stream);
     }
+  }
+  // whether the persistent kernel covers this batch: nonlinear SW, every degree 3, 5 or 7, and
+  // the same face-sum split as the per-group tiles (its 256-wide tiles split at N = 7 and not
+  // at N = 3 or 5; among the per-group tiles only the 64-wide ones at N = 5 split), so that
+  // the results stay bit-identical to the separate solves
+  bool persistent_ok() const {
+    if (kind != SELFAMD_MODEL_SW) return false;
+    for (const auto &g : groups) {
+      if (g.N != 3 && g.N != 5 && g.N != 7) return false;
+      if (g.N == 5) {
+        const int v = tile_for(kind, 5, g.nEl);
+        const int threads = v < 0 ? tile_threads<5>() : v < 12 ? 64 << (v >> 2) : 64 << ((v - 12) >> 1);
+        if (threads == 64) return false;
+      }
+    }
+    return true;
+  }
+  hipError_t launch_persistent(int stage, int tin) {
+    PersistentArgs pa{};
+    for (int k = 0; k < 3; k++)
+      for (const auto &g : groups)
+        if (g.N == kPersistentDegrees[k]) {
+          pa.g[k] = group_args(g, stage, tin);
+          pa.g[k].reverse = tin;  // per-element arithmetic does not depend on the order
+        }
+    hipLaunchKernelGGL(k_batch_persistent<PhysSW>, dim3(persistentBlocks), 256, 0, stream, pa);
+    return hipGetLastError();
+  }
+  // graphs of one RK3 step for both starting trace buffers
+  hipError_t capture() {
+    for (auto &gx : graph)
+      if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+    for (int start = 0; start < 2; start++) {
+      hipGraph_t gr;
+      hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      hipError_t le = hipSuccess;
+      if (mode == SELFAMD_BATCH_PERSISTENT) {
+        for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_persistent(st, start ^ (st & 1));
+      } else {
+        // The groups are independent, so each group's three stages run back to back (a stage
+        // then starts on the data its predecessor left in the Infinity Cache), and the group
+        // order reverses between the two graphs, which alternate step by step, so that a
+        // group's last stage of one step is followed by its first stage of the next.
+        const int ng = (int)groups.size();
+        for (int k = 0; k < ng; k++) {
+          BatchGroup &g = groups[start == 0 ? k : ng - 1 - k];
+          for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_group(g, st, start ^ (st & 1));
+        }
+      }
+      hipError_t ce = hipStreamEndCapture(stream, &gr);
+      if (le != hipSuccess) return le;
+      if (ce != hipSuccess) return ce;
+      hipError_t ie = hipGraphInstantiate(&graph[start], gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (ie != hipSuccess) return ie;
+    }
+    return hipSuccess;
   }
 };
 
@@ -2028,28 +2207,28 @@ int selfamd_batch_create(selfamd_batch **out, int nModels, selfamd_model2d *cons
       if (e) return fail("copy failed");
     }
   }
-  // capture one RK3 step of every group, for both starting trace buffers.  The groups are
-  // independent, so each group's three stages run back to back (a stage then starts on the
-  // data its predecessor left in the Infinity Cache), and the group order reverses between
-  // the two graphs, which alternate step by step, so that a group's last stage of one step is
-  // followed by its first stage of the next.
-  for (int start = 0; start < 2; start++) {
-    hipGraph_t gr;
-    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) return fail("capture failed");
-    hipError_t le = hipSuccess;
-    const int ng = (int)b->groups.size();
-    for (int k = 0; k < ng; k++) {
-      BatchGroup &g = b->groups[start == 0 ? k : ng - 1 - k];
-      for (int st = 0; st < 3; st++)
-        if (le == hipSuccess) le = b->launch_group(g, st, start ^ (st & 1));
-    }
-    hipError_t ce = hipStreamEndCapture(b->stream, &gr);
-    if (le || ce) return fail("stage launch during capture failed");
-    hipError_t ie = hipGraphInstantiate(&b->graph[start], gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    if (ie) return fail("graph instantiation failed");
-  }
+  if (b->capture() != hipSuccess) return fail("stage graph capture failed");
   *out = b;
+  return kOk;
+}
+
+int selfamd_batch_set_mode(selfamd_batch *b, int mode) {
+  SELFAMD_REQUIRE(b && (mode == SELFAMD_BATCH_PER_GROUP || mode == SELFAMD_BATCH_PERSISTENT),
+                  "batch_set_mode: bad arguments (mode=%d)", mode);
+  SELFAMD_REQUIRE(mode == SELFAMD_BATCH_PER_GROUP || b->persistent_ok(),
+                  "batch_set_mode: the persistent launch takes nonlinear SW runs of degrees 3, 5 and 7 "
+                  "whose per-group tile at N = 5 is not 64 wide (that tile splits the face sums)");
+  SELFAMD_HIP(hipStreamSynchronize(b->stream));
+  if (mode == SELFAMD_BATCH_PERSISTENT && b->persistentBlocks == 0) {
+    int dev = 0, cus = 0, perCU = 0;
+    SELFAMD_HIP(hipGetDevice(&dev));
+    SELFAMD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SELFAMD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_batch_persistent<PhysSW>, 256, 0));
+    SELFAMD_REQUIRE(cus > 0 && perCU > 0, "batch_set_mode: no resident block for the persistent kernel");
+    b->persistentBlocks = cus * perCU;
+  }
+  b->mode = mode;
+  SELFAMD_HIP(b->capture());
   return kOk;
 }
 

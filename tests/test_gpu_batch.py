@@ -44,6 +44,42 @@ def test_batch_equals_separate(kind):
         np.testing.assert_array_equal(m.solution, ma.solution)
 
 
+@pytest.mark.parametrize("periodic", [True, False])
+def test_batch_persistent_equals_separate(periodic):
+    """One persistent launch per stage for all degree groups: every run equals its separate
+    solve bit for bit, also when the mode changes between steps."""
+    specs = [(7, 5, 1), (3, 9, 2), (5, 4, 3), (7, 3, 4), (3, 6, 5), (5, 7, 6), (3, 2, 7)]
+    runs = [_run("sw", N, n, s, periodic) for N, n, s in specs]
+    alone = [_run("sw", N, n, s, periodic) for N, n, s in specs]
+    dts = [c.dt() for c, _, _ in runs]
+    b = self_amd.Batch([m for _, m, _ in runs], dts)
+    b.SetMode(self_amd.Batch.PERSISTENT)
+    b.ForwardStep(3)
+    b.SetMode(self_amd.Batch.PER_GROUP)
+    b.ForwardStep(1)
+    b.SetMode(self_amd.Batch.PERSISTENT)
+    b.ForwardStep(2)
+    b.scatter()
+    for (c, m, _), (_, ma, _), dt in zip(runs, alone, dts):
+        ma.ForwardStep(6, dt)
+        np.testing.assert_array_equal(m.solution, ma.solution)
+    c, m, _ = runs[1]
+    c.om.rk3_steps(6, dts[1])
+    assert rel_err(m.solution, c.om.solution).max() <= 1e-12
+
+
+def test_batch_persistent_rejects_other_models():
+    _, m1, _ = _run("lsw", 3, 3, 1, periodic=False)
+    b = self_amd.Batch([m1], [1e-3])
+    with pytest.raises(self_amd.SelfAmdError):
+        b.SetMode(self_amd.Batch.PERSISTENT)
+    _, m2, _ = _run("sw", 4, 3, 1)
+    b = self_amd.Batch([m2], [1e-3])
+    with pytest.raises(self_amd.SelfAmdError):
+        b.SetMode(self_amd.Batch.PERSISTENT)
+    b.ForwardStep(1)  # the batch keeps its per-group graphs
+
+
 def test_batch_rejects_mixed_kinds():
     _, m1, _ = _run("sw", 3, 3, 1)
     _, m2, _ = _run("lsw", 3, 3, 1)
@@ -51,11 +87,12 @@ def test_batch_rejects_mixed_kinds():
         self_amd.Batch([m1, m2], [1e-3, 1e-3])
 
 
-def test_full_size_ensemble():
+@pytest.mark.parametrize("mode", [self_amd.Batch.PER_GROUP, self_amd.Batch.PERSISTENT])
+def test_full_size_ensemble(mode):
     """The batched configuration as benchmarked: 512 SW runs, n x n periodic elements with
-    n ~ U{8..64} and N ~ U{3,5,7} (seed 0).  After 5 batched steps, 16 runs spread over the
-    batch equal the same runs advanced alone bit for bit, and two of them (one per end of the
-    degree range) match the oracle."""
+    n ~ U{8..64} and N ~ U{3,5,7} (seed 0), in either launch mode.  After 5 batched steps, 16
+    runs spread over the batch equal the same runs advanced alone bit for bit, and two of them
+    (one per end of the degree range) match the oracle."""
     import oracle
     from tests.cases import KIND, flatten_metrics
     rng = np.random.default_rng(0)
@@ -81,6 +118,7 @@ def test_full_size_ensemble():
     b = self_amd.Batch(models, dts)
     groups, _ = b.info()
     assert groups == 3
+    b.SetMode(mode)
     b.ForwardStep(steps)
     b.scatter()
     for r in np.linspace(0, n_runs - 1, 16).astype(int):
