@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="sw256", choices=["sw256", "euler512", "lsw10", "ensemble", "closure"])
     ap.add_argument("--closure-runs", type=int, default=8, help="fine training runs of the closure workload")
+    ap.add_argument("--closure-horizon", type=int, default=None,
+                    help="coarse steps of each closure training run (default: the judged horizon)")
+    ap.add_argument("--closure-threads", type=int, default=0,
+                    help="host threads building the closure training runs' initial states")
     ap.add_argument("--closure-aposteriori", type=int, default=1000,
                     help="steps of training through the solver (closure workload; 0: a-priori only)")
     ap.add_argument("--seed", type=int, default=0)
@@ -410,7 +414,8 @@ def run_closure(a, root):
         print(m, file=sys.stderr, flush=True)
 
     r = closure.benchmark(n_train=a.closure_runs, n_steps=a.steps or 500, log=log,
-                          aposteriori_iters=a.closure_aposteriori)
+                          aposteriori_iters=a.closure_aposteriori, train_horizon=a.closure_horizon,
+                          threads=a.closure_threads)
     runs = r["runs"]
     value = float(np.mean([x["speedup_at_equal_error"] for x in runs]))
     out = {
@@ -420,7 +425,8 @@ def run_closure(a, root):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "nonlinear SW 2D closure: coarse 64x64 periodic N=3 FP64 + FP32 MLP 51-128-128-48 "
                                "GELU, trained on fine 256x256 N=7 runs projected onto the coarse space",
-                   "train_runs": a.closure_runs, "train_samples": r["train_samples"], "alpha": r["alpha"],
+                   "train_runs": a.closure_runs, "train_horizon": a.closure_horizon or (a.steps or 500),
+                   "train_samples": r["train_samples"], "alpha": r["alpha"],
                    "skill_train": r["skill_train"], "skill_validation": r["skill_validation"],
                    "data_s": r["data_s"], "train_s": r["train_s"], "runs": runs,
                    "aposteriori": r.get("aposteriori")},

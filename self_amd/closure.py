@@ -52,9 +52,15 @@ class CoarseFine:
         self.nOut = NVAR * (Nc + 1) ** 2
 
     # --- models --------------------------------------------------------------------------
-    def fine_model(self, seed):
-        sol, topo = workloads.nonlinear_sw(self.xf, self.interp_f.controlPoints, self.interp_f.qWeights, seed=seed)
-        F = ShallowWater(self.mesh_f, self.geom_f)
+    def initial_state(self, seed):
+        """(solution, topography) of the fine run of this seed (host arrays)."""
+        return workloads.nonlinear_sw(self.xf, self.interp_f.controlPoints, self.interp_f.qWeights, seed=seed)
+
+    def fine_model(self, seed, state=None, model=None):
+        """Fine model started from the seed's initial state (or `state`); `model` is reused
+        when given (topography and solution set anew)."""
+        sol, topo = state if state is not None else self.initial_state(seed)
+        F = model if model is not None else ShallowWater(self.mesh_f, self.geom_f)
         F.SetTopography(topo)
         F.SetSolution(sol)
         return F, topo
@@ -63,9 +69,9 @@ class CoarseFine:
         """DeviceArray (coarse) of the L2 projection of a fine interior."""
         return project_l2_structured(fine, self.Nf, self.Nc, self.nxc, self.nxc, self.r, nVar)
 
-    def coarse_model(self, topo_f):
+    def coarse_model(self, topo_f, model=None):
         topo_c = self.project(topo_f.reshape(topo_f.shape[0], 1, self.Nf + 1, self.Nf + 1), 1).to_host()
-        C = ShallowWater(self.mesh_c, self.geom_c)
+        C = model if model is not None else ShallowWater(self.mesh_c, self.geom_c)
         C.SetTopography(topo_c[:, 0])
         return C
 
@@ -82,17 +88,34 @@ class CoarseFine:
         return float(np.einsum("e,ji,eji->", self.J_c, w2, S[:, 2]))
 
 
-def generate(cf, seeds, n_steps=500, stride=20, window=1, log=None):
+def generate(cf, seeds, n_steps=500, stride=20, window=1, log=None, threads=0):
     """Training pairs (raw features, raw targets) from fine runs of the given seeds: one
     sample per element at every `stride`-th coarse step of an n_steps horizon.  The target
     is the mean tendency defect over `window` coarse steps,
     (P S_f(t + w dt_c) - Phi_c^w(P S_f(t))) / (w dt_c): longer windows average out the
-    part of the defect carried by fast unresolved waves."""
+    part of the defect carried by fast unresolved waves.  With threads > 0 the initial states
+    of the next runs are built on that many host threads while the GPU advances the current
+    one, and one fine and one coarse model serve all runs."""
     assert stride >= window, "samples must not overlap (stride >= window)"
+    import collections
+    from concurrent.futures import ThreadPoolExecutor
+    seeds = list(seeds)
     X, T = [], []
-    for seed in seeds:
-        F, topo = cf.fine_model(seed)
-        C = cf.coarse_model(topo)
+    pool = ThreadPoolExecutor(threads) if threads else None
+    pending = collections.deque()
+    nxt = 0
+    Fm = Cm = None
+    for i, seed in enumerate(seeds):
+        if pool is not None:
+            while nxt < len(seeds) and len(pending) < 2 * threads:
+                pending.append(pool.submit(cf.initial_state, seeds[nxt]))
+                nxt += 1
+            F, topo = cf.fine_model(seed, state=pending.popleft().result(), model=Fm)
+            C = cf.coarse_model(topo, model=Cm)
+            Fm, Cm = F, C
+        else:
+            F, topo = cf.fine_model(seed)
+            C = cf.coarse_model(topo)
         P0, due = None, None
         for step in range(n_steps):
             if step % stride == 0 and step + window <= n_steps:
@@ -110,9 +133,11 @@ def generate(cf, seeds, n_steps=500, stride=20, window=1, log=None):
                     break
                 X.append(cf.raw_features(P0))
                 T.append(((P1 - Sc) / (window * cf.dt_c)).reshape(P0.shape[0], -1))
-        if log:
+        if log and (pool is None or (i + 1) % 100 == 0 or i + 1 == len(seeds)):
             log(f"closure data: seed {seed} done ({len(X)} snapshots)")
         del F, C
+    if pool is not None:
+        pool.shutdown()
     return np.concatenate(X), np.concatenate(T)
 
 
@@ -443,16 +468,20 @@ def aposteriori_network(cf, windows, iters, lr=1e-3, K=None, seed=0, log=None):
 
 
 def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, seeds_eval=(1000, 1001), log=None,
-              aposteriori_iters=0, aposteriori_K=8):
+              aposteriori_iters=0, aposteriori_K=8, train_horizon=None, threads=0, aposteriori_runs=8):
     """Closure workload: train on fine runs, pick the correction scale on a validation run,
     then compare on held-out runs the corrected coarse run with plain runs of increasing
     resolution.  'speedup_at_equal_error' is the wall-time of the cheapest plain run whose
     error is at most the corrected run's, divided by the corrected run's wall-time (the fine
-    run itself when no coarser plain run reaches that error)."""
+    run itself when no coarser plain run reaches that error).  Each training run covers
+    train_horizon coarse steps (default n_steps); the training through the solver takes its
+    windows from the first aposteriori_runs training runs over n_steps."""
     import time
     t0 = time.perf_counter()
     cf = CoarseFine(nxc=nxc)
-    X, T = generate(cf, range(n_train), n_steps=n_steps, stride=stride, window=window, log=log)
+    # short training runs are sampled in back-to-back windows
+    horizon, stride_t = (n_steps, stride) if train_horizon is None else (train_horizon, window)
+    X, T = generate(cf, range(n_train), n_steps=horizon, stride=stride_t, window=window, log=log, threads=threads)
     Xv, Tv = generate(cf, [500], n_steps=n_steps, stride=stride, window=window)
     gen_s = time.perf_counter() - t0
     norm = Normalisation(X, T)
@@ -481,7 +510,8 @@ def benchmark(n_train=8, n_steps=500, stride=10, window=5, epochs=10, nxc=64, se
     if aposteriori_iters:
         # training through the solver: windows of the training runs, network from a zero correction
         t2 = time.perf_counter()
-        wins = rollout_windows(cf, range(n_train), aposteriori_K, every=25, n_steps=n_steps, log=log)
+        wins = rollout_windows(cf, range(min(n_train, aposteriori_runs)), aposteriori_K, every=25, n_steps=n_steps,
+                               log=log)
         mlp_a, norm_a, hist_a = aposteriori_network(cf, wins, aposteriori_iters, K=aposteriori_K, log=log)
         weights = l2_weights(cf)
         val = rollout_windows(cf, [600], aposteriori_K, every=50, n_steps=n_steps)

@@ -191,6 +191,18 @@ def test_closure_training_gain_on_held_out_runs():
     assert abs(c["mass_drift"]) < 1e-13
 
 
+def test_generate_threaded_equals_serial():
+    """Training pairs built with host threads for the initial states and one reused fine and
+    coarse model equal those built run by run with fresh models, bit for bit."""
+    from self_amd import closure as cl
+    cf = cl.CoarseFine(nxc=8)
+    X0, T0 = cl.generate(cf, [3, 4, 5], n_steps=10, stride=5, window=5)
+    X1, T1 = cl.generate(cf, [3, 4, 5], n_steps=10, stride=5, window=5, threads=2)
+    assert X0.shape == (3 * 2 * 64, cf.nIn)
+    np.testing.assert_array_equal(X0, X1)
+    np.testing.assert_array_equal(T0, T1)
+
+
 def test_closure_at_configuration_size():
     """The closure configuration at its own size: a 256 x 256 N = 7 fine state of the seeded
     nonlinear-SW distribution projected onto the 64 x 64 N = 3 coarse space (against the
