@@ -2227,8 +2227,17 @@ int selfamd_batch_set_mode(selfamd_batch *b, int mode) {
     SELFAMD_REQUIRE(cus > 0 && perCU > 0, "batch_set_mode: no resident block for the persistent kernel");
     b->persistentBlocks = cus * perCU;
   }
+  const int previous = b->mode;
   b->mode = mode;
-  SELFAMD_HIP(b->capture());
+  const hipError_t e = b->capture();
+  if (e != hipSuccess) {
+    // keep the batch usable: back to the previous mode's graphs
+    b->mode = previous;
+    const hipError_t e2 = b->capture();
+    set_error("batch_set_mode: graph capture failed: %s%s", hipGetErrorString(e),
+              e2 == hipSuccess ? "" : " (the previous mode's graphs could not be restored either)");
+    return kHipError;
+  }
   return kOk;
 }
 
