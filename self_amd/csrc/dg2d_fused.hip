@@ -273,10 +273,17 @@ __device__ inline double lag_own(const double *un, size_t fb, double nx, double 
   if constexpr (NODAL) return un[2 * fb] * nx + un[2 * fb + 1] * ny;
   return un[fb];
 }
+// The outer side's value is formed in two steps: lag_outer_raw at the load (affine: the stored
+// value as it is, so that no arithmetic waits for the load there) and outer_un where it is used.
 template <bool NODAL>
-__device__ inline double lag_outer(const double *un, size_t fbNeighbour, double nx, double ny) {
+__device__ inline double lag_outer_raw(const double *un, size_t fbNeighbour, double nx, double ny) {
   if constexpr (NODAL) return un[2 * fbNeighbour] * nx + un[2 * fbNeighbour + 1] * ny;
-  return -un[fbNeighbour];
+  return un[fbNeighbour];
+}
+template <bool NODAL>
+__device__ inline double outer_un(double raw) {
+  if constexpr (NODAL) return raw;
+  return -raw;
 }
 
 // Ordering point between kernel phases.  When every element's node lanes and face lanes lie in
@@ -360,14 +367,16 @@ struct TileLds {
 // in rows of MRS, bM(2P), qW, 1/qW), from mats = [dgM, D, bM, qW, 1/qW] as SetupMatrices stores
 // them.  Every work-item of the block takes part.
 template <class Phys, int N, bool NODAL, int THREADS>
+__device__ __forceinline__ int tile_matrix_slot(int k) {
+  constexpr int NP = N + 1, NN = NP * NP, MRS = TileLds<Phys, N, NODAL, THREADS>::MRS;
+  const int r = k % NN;
+  return k < 2 * NN ? (k / NN) * NP * MRS + (r / NP) * MRS + r % NP : 2 * NP * MRS + k - 2 * NN;
+}
+template <class Phys, int N, bool NODAL, int THREADS>
 __device__ __forceinline__ void load_tile_matrices(const double *mats, double *lds, int tid) {
-  using L = TileLds<Phys, N, NODAL, THREADS>;
-  constexpr int NP = N + 1, NN = NP * NP, MRS = L::MRS;
-  double *sM = lds + L::M;
-  for (int k = tid; k < 2 * NN + 4 * NP; k += THREADS) {
-    const int r = k % NN;
-    sM[k < 2 * NN ? (k / NN) * NP * MRS + (r / NP) * MRS + r % NP : 2 * NP * MRS + k - 2 * NN] = mats[k];
-  }
+  double *sM = lds + TileLds<Phys, N, NODAL, THREADS>::M;
+  for (int k = tid; k < 2 * (N + 1) * (N + 1) + 4 * (N + 1); k += THREADS)
+    sM[tile_matrix_slot<Phys, N, NODAL, THREADS>(k)] = mats[k];
 }
 
 // One tile of the stage kernel: tile blk of the launch's element list (after any reordering),
@@ -417,40 +426,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
     e = __builtin_amdgcn_readfirstlane(e);
     nodeActive = __builtin_amdgcn_readfirstlane((int)nodeActive) != 0;
   }
-  const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
-
-  // ---- loads: node lanes ----
-  double s[NV], gReg[NV];
-  double auxVal = a.p.auxConst, statVal = a.p.stat2Const;
-  double dsdx[4] = {0, 0, 0, 0}, invJ = 1.0;
-  if (nodeActive) {
-#pragma unroll
-    // state and RK register: non-temporal loads.  This stage overwrites both, so caching the
-    // lines read here buys nothing; measured 5 % faster per launch than cached loads at N = 7
-    // (non-temporal loads of the static field or of the neighbour traces, which later stages
-    // or neighbours reuse, were slower).
-    for (int v = 0; v < NV; v++) s[v] = __builtin_nontemporal_load(&a.S[sBase + (size_t)v * NN + node]);
-    if (MODE == 0 && a.rkA != 0.0) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = __builtin_nontemporal_load(&a.G[sBase + (size_t)v * NN + node]);
-    } else {
-#pragma unroll
-      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
-    }
-    if (USE_AUX && a.aux) auxVal = a.aux[(size_t)e * NN + node];
-    if (a.stat2) statVal = a.stat2[(size_t)e * NN + node];
-    if constexpr (NODAL) {
-      const double *gn = a.ndsdx + 4 * ((size_t)e * NN + node);
-      dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
-      invJ = 1.0 / a.nJ[(size_t)e * NN + node];
-    } else {
-      const double *gr = a.geom + (size_t)e * kGeomStride;
-      dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
-      invJ = gr[6];  // 1/J, precomputed in the element record
-    }
-  }
-
-  // ---- loads: face lanes (neighbour traces, lagged velocities, normals) ----
+  // ---- face lanes ----
   // SPLIT: wavefront w covers the face nodes of element w (NF <= 32 slots), lanes l and l + 32
   // share face node l; otherwise face node = thread index
   // FPAD (N = 5, tiles of 128 or 256): each element's 24 face nodes take 32 lanes, south and
@@ -475,8 +451,11 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
   const int pf = blk * EPB + fel;
   const bool faceActive = ONE_FACE_PASS && (!SPLIT || fslot < NF) && (!FPAD || fslot < 2 * NP) && fel < EPB &&
                           pf < a.nEl;
-  int fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
-  if constexpr (SPLIT && NF == 32 && NN == 64) fe = __builtin_amdgcn_readfirstlane(fe);  // the wavefront's element
+  // one element per wavefront (N = 7, face lanes split in pairs): the face lanes' element is the
+  // node lanes' one, already in a scalar register
+  int fe;
+  if constexpr (SPLIT && NF == 32 && NN == 64) fe = e;
+  else fe = a.elems ? (faceActive ? a.elems[pf] : 0) : pf;
   // SPLIT: LDS slots of the half face line this lane interpolates (phases 1 and 3).  South and
   // north faces run along j through column fi, east and west along i through row fi.
   constexpr int FH0 = NP / 2, FHN = NP - FH0;
@@ -487,10 +466,75 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
 #pragma unroll
     for (int t = 0; t < FHN; t++) foff[t] = alongJ ? slot(fi, lo + t) : slot(lo + t, fi);
   }
-  double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
+  // ---- loads ----
+  // The vector-memory counter completes in issue order, and a wait after a branch must cover
+  // every path into it.  So every load is issued before any of them is waited for: the
+  // matrices (unless preloaded), the node lanes' streams (on every path: inactive lanes and
+  // absent fields read element 0 or the state, and the values are discarded), the neighbour
+  // codes (with one element per wavefront the element's four codes are one scalar load, on
+  // its own counter), then the loads addressed through them; the matrices go to LDS last.
+  constexpr int MCNT = 2 * NN + 4 * NP, MPL = PRELOADED ? 1 : (MCNT + THREADS - 1) / THREADS;
+  double mreg[MPL];
+  if constexpr (!PRELOADED) {
+#pragma unroll
+    for (int q = 0; q < MPL; q++) mreg[q] = a.mats[min(tid + q * THREADS, MCNT - 1)];
+  }
+  const size_t sBase = (size_t)e * NV * NN;  // S(0,0,0,e)
+  const size_t nodeAt = nodeActive ? (size_t)e * NN + node : (size_t)node;  // inactive: element 0
+  double s[NV], gReg[NV];
+  double auxVal = a.p.auxConst, statVal = a.p.stat2Const;
+  double dsdx[4] = {0, 0, 0, 0}, invJ = 1.0;
+  {
+    const size_t sAt = nodeActive ? sBase + node : (size_t)node;
+    // state and RK register: non-temporal loads.  This stage overwrites both, so caching the
+    // lines read here buys nothing; measured 5 % faster per launch than cached loads at N = 7
+    // (non-temporal loads of the static field or of the neighbour traces, which later stages
+    // or neighbours reuse, were slower).
+#pragma unroll
+    for (int v = 0; v < NV; v++) s[v] = __builtin_nontemporal_load(&a.S[sAt + (size_t)v * NN]);
+    if constexpr (MODE == 0) {
+      // the first stage (rkA = 0) reads element 0's, a few cached lines, and discards them
+      const bool useG = a.rkA != 0.0;
+      const size_t gAt = useG ? sAt : (size_t)node;
+      double gl[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) gl[v] = __builtin_nontemporal_load(&a.G[gAt + (size_t)v * NN]);
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = useG ? gl[v] : 0.0;
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; v++) gReg[v] = 0.0;
+    }
+    if constexpr (USE_AUX) {
+      const double av = a.aux ? a.aux[nodeAt] : a.S[node];
+      if (a.aux) auxVal = av;
+    }
+    const double sv = a.stat2 ? a.stat2[nodeAt] : a.S[node];
+    if (a.stat2) statVal = sv;
+    if constexpr (NODAL) {
+      const double *gn = a.ndsdx + 4 * nodeAt;
+      dsdx[0] = gn[0]; dsdx[1] = gn[1]; dsdx[2] = gn[2]; dsdx[3] = gn[3];
+      invJ = 1.0 / a.nJ[nodeAt];
+    } else {
+      const double *gr = a.geom + (nodeActive ? (size_t)e : 0) * kGeomStride;
+      dsdx[0] = gr[0]; dsdx[1] = gr[1]; dsdx[2] = gr[2]; dsdx[3] = gr[3];
+      invJ = gr[6];  // 1/J, precomputed in the element record
+    }
+  }
   int code = 0;
+  if constexpr (SPLIT && NF == 32 && NN == 64) {
+    const int4 c4 = *reinterpret_cast<const int4 *>(a.nbr + 4 * (size_t)fe);
+    // all four in scalar registers at once (one load, one wait), then a per-lane select
+    const int cx = __builtin_amdgcn_readfirstlane(c4.x), cy = __builtin_amdgcn_readfirstlane(c4.y);
+    const int cz = __builtin_amdgcn_readfirstlane(c4.z), cw = __builtin_amdgcn_readfirstlane(c4.w);
+    code = (side & 2) ? ((side & 1) ? cw : cz) : ((side & 1) ? cy : cx);
+    if (!(faceActive && fhalf == 0)) code = 0;
+  } else {
+    if (faceActive && fhalf == 0) code = a.nbr[4 * (size_t)fe + side];
+  }
+
+  double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
   if (faceActive && fhalf == 0) {
-    code = a.nbr[4 * (size_t)fe + side];
     // normals: first in nodal mode, where the lagged normal velocities need them; after the
     // neighbour loads in affine mode (measured slightly faster there)
     if constexpr (NODAL) {
@@ -504,7 +548,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
       const double *tr = a.traceIn + (size_t)(flip ? N - fi : fi) + NP * (size_t)NT * (s2 + 4 * e2);
 #pragma unroll
       for (int k = 0; k < NT; k++) qR[k] = tr[(size_t)k * NP];
-      if (Phys::kLag) unR = lag_outer<NODAL>(a.unIn, nb, nx, ny);
+      if (Phys::kLag) unR = lag_outer_raw<NODAL>(a.unIn, nb, nx, ny);
     }
     if (Phys::kLag) unL = lag_own<NODAL>(a.unIn, (size_t)fi + NP * (side + 4 * (size_t)fe), nx, ny);
     if constexpr (!NODAL) {
@@ -512,7 +556,15 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
       nx = gr[8 + side]; ny = gr[12 + side]; nmag = gr[16 + side];
     }
   }
-  if constexpr (!PRELOADED) load_tile_matrices<Phys, N, NODAL, THREADS>(a.mats, lds, tid);
+
+  // the matrices into LDS once every load is in flight
+  if constexpr (!PRELOADED) {
+#pragma unroll
+    for (int q = 0; q < MPL; q++) {
+      const int k = tid + q * THREADS;
+      if (k < MCNT) sM[tile_matrix_slot<Phys, N, NODAL, THREADS>(k)] = mreg[q];
+    }
+  }
   const double *dgM = sM, *Dm = sM + NP * MRS, *bM0 = sM + 2 * NP * MRS, *bM1 = bM0 + NP, *qW = bM1 + NP;
 
   // ---- phase 0: volume fluxes, nodal trace quantities ----
@@ -611,7 +663,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
       }
     }
     double fstar[NV];
-    Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, unR_, a.p, fstar);
+    Phys::riemann(qL, qR_, nx_, ny_, nmag_, auxB, unL_, outer_un<NODAL>(unR_), a.p, fstar);
 #pragma unroll
     for (int v = 0; v < NV; v++) sBF[v][fel_][fnode_] = fstar[v];
   };
@@ -638,7 +690,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
         const size_t e2 = (size_t)(code2 >> 3);
         const double *tr = a.traceIn + (size_t)(flip ? N - fi2 : fi2) + NP * (size_t)NT * (s2 + 4 * e2);
         for (int k = 0; k < NT; k++) qR2[k] = tr[(size_t)k * NP];
-        if (Phys::kLag) unR2 = lag_outer<NODAL>(a.unIn, (size_t)(flip ? N - fi2 : fi2) + NP * (s2 + 4 * e2), nx2, ny2);
+        if (Phys::kLag) unR2 = lag_outer_raw<NODAL>(a.unIn, (size_t)(flip ? N - fi2 : fi2) + NP * (s2 + 4 * e2), nx2, ny2);
       }
       if (Phys::kLag) unL2 = lag_own<NODAL>(a.unIn, (size_t)fi2 + NP * (side2 + 4 * (size_t)fe2), nx2, ny2);
       face(fel2, side2, fi2, fe2, fnode2, code2, qR2, unL2, unR2, nx2, ny2, nmag2);
