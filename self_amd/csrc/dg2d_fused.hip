@@ -469,10 +469,19 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
   // ---- loads ----
   // The vector-memory counter completes in issue order, and a wait after a branch must cover
   // every path into it.  So every load is issued before any of them is waited for: the
-  // matrices (unless preloaded), the node lanes' streams (on every path: inactive lanes and
-  // absent fields read element 0 or the state, and the values are discarded), the neighbour
-  // codes (with one element per wavefront the element's four codes are one scalar load, on
-  // its own counter), then the loads addressed through them; the matrices go to LDS last.
+  // neighbour codes (first, since the trace loads depend on them), the matrices (unless
+  // preloaded), the node lanes' streams (on every path: inactive lanes and absent fields read
+  // element 0 or the state, and the values are discarded), then the loads addressed through
+  // the codes, which wait for the codes alone; the matrices go to LDS last.
+  // one element per wavefront (N = 7, face lanes split in pairs): the element's four codes are
+  // one scalar load further down; otherwise each face lane's code is the first vector load
+  constexpr bool SCALAR_CODES = SPLIT && NF == 32 && NN == 64;
+  int code = 0;
+  if constexpr (!SCALAR_CODES) {
+    const bool own = faceActive && fhalf == 0;
+    const int c = a.nbr[4 * (size_t)(own ? fe : 0) + side];
+    code = own ? c : 0;
+  }
   constexpr int MCNT = 2 * NN + 4 * NP, MPL = PRELOADED ? 1 : (MCNT + THREADS - 1) / THREADS;
   double mreg[MPL];
   if constexpr (!PRELOADED) {
@@ -521,16 +530,13 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
       invJ = gr[6];  // 1/J, precomputed in the element record
     }
   }
-  int code = 0;
-  if constexpr (SPLIT && NF == 32 && NN == 64) {
+  if constexpr (SCALAR_CODES) {
     const int4 c4 = *reinterpret_cast<const int4 *>(a.nbr + 4 * (size_t)fe);
     // all four in scalar registers at once (one load, one wait), then a per-lane select
     const int cx = __builtin_amdgcn_readfirstlane(c4.x), cy = __builtin_amdgcn_readfirstlane(c4.y);
     const int cz = __builtin_amdgcn_readfirstlane(c4.z), cw = __builtin_amdgcn_readfirstlane(c4.w);
     code = (side & 2) ? ((side & 1) ? cw : cz) : ((side & 1) ? cy : cx);
     if (!(faceActive && fhalf == 0)) code = 0;
-  } else {
-    if (faceActive && fhalf == 0) code = a.nbr[4 * (size_t)fe + side];
   }
 
   double qR[NT], unL = 0.0, unR = 0.0, nx = 0.0, ny = 0.0, nmag = 0.0;
