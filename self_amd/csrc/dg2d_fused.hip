@@ -531,7 +531,9 @@ __device__ __forceinline__ void fused_tile(const FusedArgs &a, int blk, double *
     }
   }
   if constexpr (SCALAR_CODES) {
-    const int4 c4 = *reinterpret_cast<const int4 *>(a.nbr + 4 * (size_t)fe);
+    // (a wavefront past the last element of a ragged last tile reads element 0's codes; the
+    // wavefront's node and face lanes are the same element, so nodeActive, a scalar, decides)
+    const int4 c4 = *reinterpret_cast<const int4 *>(a.nbr + 4 * (size_t)(nodeActive ? fe : 0));
     // all four in scalar registers at once (one load, one wait), then a per-lane select
     const int cx = __builtin_amdgcn_readfirstlane(c4.x), cy = __builtin_amdgcn_readfirstlane(c4.y);
     const int cz = __builtin_amdgcn_readfirstlane(c4.z), cw = __builtin_amdgcn_readfirstlane(c4.w);
