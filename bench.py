@@ -110,13 +110,15 @@ def copy_bandwidth(self_amd, nbytes=1 << 30, reps=20):
 # GPU clocks drop after the GPU idles for about 10 ms or more and take about 25 ms of load to
 # come back: the first 20 SW steps after such a gap run at 131-137 us per stage launch, the
 # next 20 at ~124, then ~113 (one box, 256^2 N = 7).  Setup leaves the GPU idle, so the
-# warm-up steps are queued behind this much streaming work on the model's stream; the timed
-# steps then run at the clocks of a sustained run.
+# warm-up steps are queued behind this much streaming work on the model's stream.
 SETTLE_COPIES = 250  # 1 GiB copies, ~0.1 s at 5 TB/s
-# The streaming copies bring the first 20 steps from ~137 to ~118 us per launch; only the
-# workload itself takes them to the sustained ~108-111 us (no other settling work measured did:
-# FP64 per-operator kernels 120 us, copies and kernels mixed 119 us).  After the reported
-# measurement the bench therefore also times the same steps after this many further steps.
+# The streaming copies bring the first 20 steps to ~107-109 us per launch; only the workload
+# itself takes them to the sustained ~100-103 us (no other settling work measured did: FP64
+# per-operator kernels, copies and kernels mixed).  So after the copies the bench runs this many
+# untimed steps of the workload itself, then the caller's warm-up steps, then the timed steps.
+SETTLE_STEPS = 100
+# After the reported measurement the same steps are timed again this many steps later, as a
+# check that the reported window ran at the sustained rate.
 SUSTAIN_STEPS = 100
 
 
@@ -266,6 +268,7 @@ def run_single(a, root):
     steps = a.steps or w["steps"]
     dof = w["mesh"].nElem * (N + 1) ** 2 * w["nVar"]
     settle = clock_settle(self_amd, model.stream)
+    model.ForwardStep(SETTLE_STEPS, dt)
     model.ForwardStep(a.warmup, dt)
     self_amd.synchronize()
     t0 = time.perf_counter()
@@ -280,12 +283,12 @@ def run_single(a, root):
     launch_s = sec / (3 * steps)
     bpd = alg_bytes_per_dof_step(N, w["nVar"], w["NT"], w["p"])
     achieved = bpd / 3.0 * dof / launch_s / 1e9
-    # after the reported measurement: the same kernel once the GPU has run it for SUSTAIN_STEPS
-    # more steps (its clocks then settled by this workload itself), reported beside the value
+    # after the reported measurement: the same steps timed again SUSTAIN_STEPS steps later,
+    # reported beside the value
     model.ForwardStep(SUSTAIN_STEPS, dt)
     self_amd.synchronize()
     sus_launch_s = model.time_steps(steps, dt) / 1e3 / (3 * steps)
-    sustained = {"after_steps": a.warmup + steps + SUSTAIN_STEPS, "timed_steps": steps,
+    sustained = {"after_steps": SETTLE_STEPS + a.warmup + steps + SUSTAIN_STEPS, "timed_steps": steps,
                  "avg_launch_us": sus_launch_s * 1e6,
                  "frac": bpd / 3.0 * dof / sus_launch_s / 1e9 / HBM_PEAK_GBS,
                  "value": dof / (3 * sus_launch_s)}
@@ -312,7 +315,8 @@ def run_single(a, root):
         "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall,
                    "metrics": ("per-node metric terms (SEMQuad)" if a.nodal_metrics
                                else "affine, one record per element"),
-                   "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB queued ahead of the warm-up steps"},
+                   "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB, then {SETTLE_STEPS} untimed "
+                                   f"steps of the workload, ahead of the warm-up steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
                      "avg_launch_us": launch_s * 1e6, "bytes_per_launch_alg": bpd / 3.0 * dof,
