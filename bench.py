@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--closure-aposteriori", type=int, default=1000,
                     help="steps of training through the solver (closure workload; 0: a-priori only)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--settle-steps", type=int, default=None,
+                    help="untimed steps of the workload after the clock-settling copies, before the "
+                         "warm-up steps (default: 100 for single runs, 10 for the ensemble)")
     ap.add_argument("--batch-mode", default="per-group", choices=["per-group", "persistent"],
                     help="ensemble: one launch per degree group and stage, or one persistent launch per stage")
     ap.add_argument("--cpu-steps", type=int, default=None, help="RK3 steps of the CPU baseline sample")
@@ -117,6 +120,7 @@ SETTLE_COPIES = 250  # 1 GiB copies, ~0.1 s at 5 TB/s
 # per-operator kernels, copies and kernels mixed).  So after the copies the bench runs this many
 # untimed steps of the workload itself, then the caller's warm-up steps, then the timed steps.
 SETTLE_STEPS = 100
+ENSEMBLE_SETTLE_STEPS = 10  # ~30 ms of the 512-run batch, like 100 steps of the 256^2 run
 # After the reported measurement the same steps are timed again this many steps later, as a
 # check that the reported window ran at the sustained rate.
 SUSTAIN_STEPS = 100
@@ -267,8 +271,9 @@ def run_single(a, root):
         return traffic_probe(self_amd, model, dt)
     steps = a.steps or w["steps"]
     dof = w["mesh"].nElem * (N + 1) ** 2 * w["nVar"]
+    settle_steps = SETTLE_STEPS if a.settle_steps is None else a.settle_steps
     settle = clock_settle(self_amd, model.stream)
-    model.ForwardStep(SETTLE_STEPS, dt)
+    model.ForwardStep(settle_steps, dt)
     model.ForwardStep(a.warmup, dt)
     self_amd.synchronize()
     t0 = time.perf_counter()
@@ -288,7 +293,7 @@ def run_single(a, root):
     model.ForwardStep(SUSTAIN_STEPS, dt)
     self_amd.synchronize()
     sus_launch_s = model.time_steps(steps, dt) / 1e3 / (3 * steps)
-    sustained = {"after_steps": SETTLE_STEPS + a.warmup + steps + SUSTAIN_STEPS, "timed_steps": steps,
+    sustained = {"after_steps": settle_steps + a.warmup + steps + SUSTAIN_STEPS, "timed_steps": steps,
                  "avg_launch_us": sus_launch_s * 1e6,
                  "frac": bpd / 3.0 * dof / sus_launch_s / 1e9 / HBM_PEAK_GBS,
                  "value": dof / (3 * sus_launch_s)}
@@ -315,7 +320,7 @@ def run_single(a, root):
         "config": {"workload": w["desc"], "elements": w["mesh"].nElem, "N": N, "dof": dof, "dt": dt, "wall_s": wall,
                    "metrics": ("per-node metric terms (SEMQuad)" if a.nodal_metrics
                                else "affine, one record per element"),
-                   "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB, then {SETTLE_STEPS} untimed "
+                   "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB, then {settle_steps} untimed "
                                    f"steps of the workload, ahead of the warm-up steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
@@ -365,8 +370,9 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
     steps = a.steps or 20
     # the batch runs on its own stream: the settling copies run beside the warm-up steps on the
     # default stream, so the GPU stays busy up to the timed steps
+    settle_steps = ENSEMBLE_SETTLE_STEPS if a.settle_steps is None else a.settle_steps
     settle = clock_settle(self_amd, None)
-    batch.ForwardStep(a.warmup)
+    batch.ForwardStep(settle_steps + a.warmup)
     self_amd.synchronize()
     ms = batch.time_steps(steps)
     del settle
@@ -379,7 +385,7 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
         m = self_amd.ShallowWater(mesh, geom)
         m.SetTopography(inits[r][1])
         m.SetSolution(inits[r][0])
-        m.ForwardStep(a.warmup + steps, dts[r])
+        m.ForwardStep(settle_steps + a.warmup + steps, dts[r])
         ref, got = m.solution, models[r].solution
         errs.append(float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))))
     bytes_step = 0.0
@@ -397,7 +403,9 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
                                + ("one persistent launch per RK stage for all degree groups"
                                   if persistent else "one fused launch per degree group and RK stage")
                                + ", one hipGraph per step", "batch_mode": a.batch_mode, "runs": n_runs, "groups": groups, "dof": dof, "setup_s": setup_s,
-                   "max_rel_err_vs_separate": max(errs), "runs_checked": n_check},
+                   "max_rel_err_vs_separate": max(errs), "runs_checked": n_check,
+                   "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB beside {settle_steps} untimed "
+                                   f"steps of the batch and the warm-up steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_batch_persistent<PhysSW>" if persistent
