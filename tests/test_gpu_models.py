@@ -180,6 +180,19 @@ def test_banded_steps_bit_identical(kind, n, periodic, bands):
     assert np.array_equal(a.solution, b.solution)
 
 
+@pytest.mark.parametrize("N,nx,ny", [(7, 91, 91), (5, 90, 92), (3, 91, 91)])
+def test_ragged_last_tile(N, nx, ny):
+    """Meshes just above the small-launch size take the tuned tiles (4, 7 and 4 elements per
+    block at N = 7, 5, 3), and these element counts leave the last tile partly empty: the
+    inactive lanes and wavefronts of that tile must neither change nor read past the arrays."""
+    c = Case("sw", N, nx, ny, periodic=True, seed=12, metrics="affine")
+    m, _ = c.gpu_model()
+    dt = c.dt()
+    m.ForwardStep(2, dt)
+    c.om.rk3_steps(2, dt)
+    assert rel_err(m.solution, c.om.solution).max() <= TOL
+
+
 def test_hopr_block_mesh_parity():
     import json
     import os
