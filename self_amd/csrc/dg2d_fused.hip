@@ -1233,12 +1233,17 @@ struct selfamd_model2d {
   int cur = 0;  // trace buffer holding the traces of the current S
   Params p{};
   double t = 0.0;
-  // captured RK3 step graphs, one per starting trace buffer
+  // captured one-step graphs, one per starting trace buffer
   hipGraphExec_t graph[2] = {nullptr, nullptr};
-  // kGraphSteps steps per graph (an even number of stage flips: same start parity after),
-  // for long runs of small meshes, where per-graph launch overhead shows
-  static constexpr int kGraphSteps = 8;
-  hipGraphExec_t graphK[2] = {nullptr, nullptr};
+  // graphs of 2^k steps (k = 1 .. top_level()), per starting trace buffer, captured (for both
+  // buffers) when a call first needs them; 2^k steps flip the trace buffer an even number of
+  // times, so the start parity is unchanged after each.  A call replays the largest as often as it fits, then each
+  // smaller one at most once and, for an odd count, the one-step graph.  Between two replays
+  // the GPU idles ~9 us, a tenth of a stage launch on small meshes.
+  static constexpr int kGraphLevels = 7;        // up to 64 steps per graph
+  static constexpr int kGraphLevelsBanded = 4;  // banded stepping: up to 8 (many more nodes per step)
+  hipGraphExec_t graphP[kGraphLevels][2] = {};
+  int top_level() const { return (nBands > 0 ? kGraphLevelsBanded : kGraphLevels) - 1; }
   double graphDt = -1.0;
   int graphEpoch = -1;  // tile table version the graphs were captured with
   ClosureBuffers *closure = nullptr;
@@ -1305,8 +1310,9 @@ struct selfamd_model2d {
   void drop_graphs() {
     for (auto &g : graph)
       if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-    for (auto &g : graphK)
-      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto &lv : graphP)
+      for (auto &g : lv)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     graphDt = -1.0;
   }
   // captures `steps` RK3 steps starting from trace buffer `start`
@@ -1336,6 +1342,9 @@ struct selfamd_model2d {
     if (ce != hipSuccess) return ce;
     hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
+    // the graph's first replay would otherwise carry its upload to the device (~25 us at 8
+    // steps), inside whatever timed region first needs it
+    if (ie == hipSuccess) ie = hipGraphUpload(*out, stream);
     return ie;
   }
   // `steps` RK3 steps launched stage by stage on the stream, without a graph
@@ -1761,8 +1770,20 @@ hipError_t prepare_graphs(selfamd_model2d *m, int nSteps, double dt) {
     m->graphDt = dt;
     m->graphEpoch = g_tileEpoch;
   }
-  const int K = selfamd_model2d::kGraphSteps;
-  if (nSteps >= K && !m->graphK[m->cur]) return m->capture(m->cur, K, dt, &m->graphK[m->cur]);
+  // the 2^k-step graphs of nSteps' decomposition (they keep the current buffer), for both
+  // starting buffers, so that a later call of the same size finds them whichever is current
+  const int top = m->top_level();
+  int rem = nSteps;
+  for (int k = top; k >= 1; k--) {
+    const int s = 1 << k;
+    if (rem < s) continue;
+    for (int start = 0; start < 2; start++)
+      if (!m->graphP[k][start]) {
+        const hipError_t e = m->capture(start, s, dt, &m->graphP[k][start]);
+        if (e != hipSuccess) return e;
+      }
+    rem = k == top ? rem % s : rem - s;
+  }
   return hipSuccess;
 }
 }  // namespace
@@ -1775,13 +1796,18 @@ int selfamd_model2d_forward_step(selfamd_model2d *m, int nSteps, double dt) {
     return kOk;
   }
   SELFAMD_HIP(prepare_graphs(m, nSteps, dt));
-  const int K = selfamd_model2d::kGraphSteps;
-  int n = 0;
-  for (; n + K <= nSteps; n += K) {
-    SELFAMD_HIP(hipGraphLaunch(m->graphK[m->cur], m->stream));  // 3K stage flips: parity unchanged
-    for (int q = 0; q < K; q++) m->t += dt;
+  const int top = m->top_level();
+  int n = nSteps;
+  for (int k = top; k >= 1; k--) {
+    const int s = 1 << k;
+    while (n >= s) {
+      SELFAMD_HIP(hipGraphLaunch(m->graphP[k][m->cur], m->stream));  // 3s stage flips: parity unchanged
+      for (int q = 0; q < s; q++) m->t += dt;
+      n -= s;
+      if (k != top) break;
+    }
   }
-  for (; n < nSteps; n++) {
+  if (n == 1) {
     SELFAMD_HIP(hipGraphLaunch(m->graph[m->cur], m->stream));
     m->cur ^= 1;  // three stages flip the trace buffer three times
     m->t += dt;
@@ -2164,6 +2190,7 @@ struct selfamd_batch {
       if (ce != hipSuccess) return ce;
       hipError_t ie = hipGraphInstantiate(&graph[start], gr, nullptr, nullptr, 0);
       (void)hipGraphDestroy(gr);
+      if (ie == hipSuccess) ie = hipGraphUpload(graph[start], stream);  // not on the first replay
       if (ie != hipSuccess) return ie;
     }
     return hipSuccess;

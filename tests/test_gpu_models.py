@@ -112,18 +112,22 @@ def test_tendency_then_steps_keeps_lag_state():
 
 @pytest.mark.parametrize("kind", ["sw", "euler"])
 def test_multi_step_graph_matches_single_steps(kind):
-    """Long runs replay an 8-step graph; the state (and model time) must be the same as after
-    the same steps replayed one at a time, and match the oracle."""
+    """Long runs replay graphs of 2, 4, ..., 64 steps (the largest as often as it fits, each
+    smaller one at most once, then the one-step graph for an odd count); the state (and model
+    time) must be the same as after the same steps replayed one at a time, and match the
+    oracle.  27 = 16+8+2+1 leaves the other trace buffer current, so 101 = 64+32+4+1 and
+    66 = 64+2 run the graphs captured from it."""
     c = Case(kind, 5, 4, 4, periodic=(kind == "sw"), seed=4, metrics="affine")
     a, _ = c.gpu_model()
     b, _ = c.gpu_model()
     dt = c.dt()
-    a.ForwardStep(19, dt)
-    for _ in range(19):
-        b.ForwardStep(1, dt)
-    assert np.array_equal(a.solution, b.solution)
-    assert a.t == b.t
-    c.om.rk3_steps(19, dt)
+    for n in (27, 101, 66):
+        a.ForwardStep(n, dt)
+        for _ in range(n):
+            b.ForwardStep(1, dt)
+        assert np.array_equal(a.solution, b.solution), n
+        assert a.t == b.t
+    c.om.rk3_steps(27 + 101 + 66, dt)
     assert rel_err(a.solution, c.om.solution).max() <= TOL
 
 
