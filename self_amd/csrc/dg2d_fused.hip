@@ -2107,7 +2107,12 @@ struct selfamd_batch {
   bool anyBoundary = false;  // some run's mesh has a physical boundary side
   hipStream_t stream = nullptr;
   int cur = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step, per starting trace buffer
+  // graphs of 2^k steps (k = 1 .. kLevels - 1), per starting buffer, captured when a call first
+  // needs them (2^k steps keep the starting buffer); a call replays the largest as often as it
+  // fits and each smaller one at most once, as the single-run engine does
+  static constexpr int kLevels = 5;  // up to 16 steps per graph
+  hipGraphExec_t graphP[kLevels][2] = {};
 
   // SELFAMD_BATCH_PER_GROUP: one launch per degree group and stage; SELFAMD_BATCH_PERSISTENT:
   // one persistent launch per stage for all groups (SW, degrees 3, 5, 7)
@@ -2164,34 +2169,65 @@ struct selfamd_batch {
     return hipGetLastError();
   }
   // graphs of one RK3 step for both starting trace buffers
-  hipError_t capture() {
-    for (auto &gx : graph)
-      if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
-    for (int start = 0; start < 2; start++) {
-      hipGraph_t gr;
-      hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) return e;
-      hipError_t le = hipSuccess;
+  // captures `steps` steps starting from trace buffer `start` (each step flips the buffer)
+  hipError_t capture_steps(int start, int steps, hipGraphExec_t *out) {
+    hipGraph_t gr;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = hipSuccess;
+    for (int n = 0; n < steps && le == hipSuccess; n++) {
+      const int tin = start ^ (n & 1);
       if (mode == SELFAMD_BATCH_PERSISTENT) {
-        for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_persistent(st, start ^ (st & 1));
+        for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_persistent(st, tin ^ (st & 1));
       } else {
         // The groups are independent, so each group's three stages run back to back (a stage
         // then starts on the data its predecessor left in the Infinity Cache), and the group
-        // order reverses between the two graphs, which alternate step by step, so that a
-        // group's last stage of one step is followed by its first stage of the next.
+        // order reverses from step to step (with the buffer parity), so that a group's last
+        // stage of one step is followed by its first stage of the next.
         const int ng = (int)groups.size();
         for (int k = 0; k < ng; k++) {
-          BatchGroup &g = groups[start == 0 ? k : ng - 1 - k];
-          for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_group(g, st, start ^ (st & 1));
+          BatchGroup &g = groups[tin == 0 ? k : ng - 1 - k];
+          for (int st = 0; st < 3 && le == hipSuccess; st++) le = launch_group(g, st, tin ^ (st & 1));
         }
       }
-      hipError_t ce = hipStreamEndCapture(stream, &gr);
-      if (le != hipSuccess) return le;
-      if (ce != hipSuccess) return ce;
-      hipError_t ie = hipGraphInstantiate(&graph[start], gr, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(gr);
-      if (ie == hipSuccess) ie = hipGraphUpload(graph[start], stream);  // not on the first replay
-      if (ie != hipSuccess) return ie;
+    }
+    hipError_t ce = hipStreamEndCapture(stream, &gr);
+    if (le != hipSuccess) return le;
+    if (ce != hipSuccess) return ce;
+    hipError_t ie = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (ie == hipSuccess) ie = hipGraphUpload(*out, stream);  // not on the first replay
+    return ie;
+  }
+  void drop_graphs() {
+    for (auto &gx : graph)
+      if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+    for (auto &lv : graphP)
+      for (auto &gx : lv)
+        if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+  }
+  // the one-step graphs of the current mode (the longer ones follow when first needed)
+  hipError_t capture() {
+    drop_graphs();
+    for (int start = 0; start < 2; start++) {
+      const hipError_t e = capture_steps(start, 1, &graph[start]);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  // the 2^k-step graphs of nSteps' decomposition, for both starting buffers
+  hipError_t prepare(int nSteps) {
+    const int top = kLevels - 1;
+    int rem = nSteps;
+    for (int k = top; k >= 1; k--) {
+      const int s = 1 << k;
+      if (rem < s) continue;
+      for (int start = 0; start < 2; start++)
+        if (!graphP[k][start]) {
+          const hipError_t e = capture_steps(start, s, &graphP[k][start]);
+          if (e != hipSuccess) return e;
+        }
+      rem = k == top ? rem % s : rem - s;
     }
     return hipSuccess;
   }
@@ -2202,8 +2238,7 @@ extern "C" {
 int selfamd_batch_destroy(selfamd_batch *b) {
   if (!b) return kOk;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (auto &gx : b->graph)
-    if (gx) (void)hipGraphExecDestroy(gx);
+  b->drop_graphs();
   for (auto &g : b->groups) {
     double *bufs[] = {g.S, g.G, g.trace[0], g.trace[1], g.un[0], g.un[1], g.geom, g.mats, g.aux, g.stat2, g.presTrace};
     for (double *p : bufs)
@@ -2330,7 +2365,18 @@ int selfamd_batch_set_mode(selfamd_batch *b, int mode) {
 
 int selfamd_batch_forward_step(selfamd_batch *b, int nSteps) {
   SELFAMD_REQUIRE(b && nSteps >= 0, "batch_forward_step: bad arguments");
-  for (int n = 0; n < nSteps; n++) {
+  SELFAMD_HIP(b->prepare(nSteps));
+  const int top = selfamd_batch::kLevels - 1;
+  int n = nSteps;
+  for (int k = top; k >= 1; k--) {
+    const int s = 1 << k;
+    while (n >= s) {
+      SELFAMD_HIP(hipGraphLaunch(b->graphP[k][b->cur], b->stream));  // an even step count: same buffer
+      n -= s;
+      if (k != top) break;
+    }
+  }
+  if (n == 1) {
     SELFAMD_HIP(hipGraphLaunch(b->graph[b->cur], b->stream));
     b->cur ^= 1;
   }
@@ -2339,7 +2385,9 @@ int selfamd_batch_forward_step(selfamd_batch *b, int nSteps) {
 }
 
 int selfamd_batch_time_steps(selfamd_batch *b, int nSteps, double *elapsedMs) {
-  SELFAMD_REQUIRE(b && elapsedMs, "batch_time_steps: null argument");
+  SELFAMD_REQUIRE(b && elapsedMs && nSteps >= 0, "batch_time_steps: bad arguments");
+  // graph capture is host work: done before the first event
+  SELFAMD_HIP(b->prepare(nSteps));
   hipEvent_t e0, e1;
   SELFAMD_HIP(hipEventCreate(&e0));
   SELFAMD_HIP(hipEventCreate(&e1));
