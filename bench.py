@@ -458,7 +458,7 @@ def run_closure(a, root):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+        print(f"bench.py runs on one GPU only: the engine has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
     root = os.path.dirname(os.path.abspath(__file__))
