@@ -414,7 +414,10 @@ def run_ensemble(a, root, n_runs=512, n_check=16):
                    "clock_settle": f"{SETTLE_COPIES} streaming copies of 1 GiB beside {settle_steps} untimed "
                                    f"steps of the batch and the warm-up steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     # measured HBM bytes of one RK stage of all degree groups (per-group mode)
+                     "traffic": None if persistent else traffic_from_profiles(root, "sw2d_ensemble512_stage_traffic.json"),
+                     "traffic_scope": None if persistent else "one RK stage, all degree groups (three launches)",
                      "kernel": "k_batch_persistent<PhysSW>" if persistent
                      else "k_fused<PhysSW, N, 0, false, T> (one per degree group)"},
         "cpu_baseline": None,
