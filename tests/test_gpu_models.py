@@ -116,18 +116,18 @@ def test_multi_step_graph_matches_single_steps(kind):
     smaller one at most once, then the one-step graph for an odd count); the state (and model
     time) must be the same as after the same steps replayed one at a time, and match the
     oracle.  27 = 16+8+2+1 leaves the other trace buffer current, so 101 = 64+32+4+1 and
-    66 = 64+2 run the graphs captured from it."""
+    66 = 64+2 run the graphs captured from it; 130 = 64+64+2 replays the largest twice."""
     c = Case(kind, 5, 4, 4, periodic=(kind == "sw"), seed=4, metrics="affine")
     a, _ = c.gpu_model()
     b, _ = c.gpu_model()
     dt = c.dt()
-    for n in (27, 101, 66):
+    for n in (27, 101, 66, 130):
         a.ForwardStep(n, dt)
         for _ in range(n):
             b.ForwardStep(1, dt)
         assert np.array_equal(a.solution, b.solution), n
         assert a.t == b.t
-    c.om.rk3_steps(27 + 101 + 66, dt)
+    c.om.rk3_steps(27 + 101 + 66 + 130, dt)
     assert rel_err(a.solution, c.om.solution).max() <= TOL
 
 
